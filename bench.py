@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched Kronecker mat-vec on one MI355X.
+
+A step is one call of the library over the whole batch (one kernel launch). The default workload is the
+d=6, n=4 ASGarD-like operator apply in FP64 (8192 entries: 128 output elements x 64 input elements; six
+64x64 coefficient matrices whose 4x4 blocks are the factors, stride 64). Inputs are resident in HBM before
+the timed region; the outputs accumulate across steps as the operation specifies.
+
+Printed: one JSON line with GFLOP/s (value), the roofline of the kernel (contract bytes per launch over the
+average kernel duration, measured with HIP events on the launch stream, against 8 TB/s HBM; and FP64 flops
+against 78.6 TFLOP/s), and the reference's CPU implementation timed on this host.
+
+    python bench.py [--steps K] [--warmup W] [--config c2] [--dtype f64] [--cpu-baseline auto|off]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+WORKLOADS = {
+    "c1": "d=3 n=4 stride=4, nb=4096 entries with own factors, 512 outputs x 8 contributions",
+    "c2": ("ASGarD-like 6D operator apply: d=6 n=4 stride=64, 128 output x 64 input elements (nb=8192), "
+           "factors = digit-indexed 4x4 blocks of six 64x64 N(0,1)/8 matrices, 64 shared inputs of 4096, "
+           "128 outputs x 64 contributions"),
+    "c4": "d=4 n=8 stride=8, nb=16384, factors U[-1,1]*10^U[-2,2], 1024 outputs x 16 contributions",
+    "c5": "d=3 n=8 stride=8, nb=8192, 256 shared factor triples, 2048 outputs x 4 contributions",
+}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_threads() -> int:
+    env = os.environ.get("OMP_NUM_THREADS")
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    if env and env.isdigit() and int(env) > 0:
+        return min(int(env), avail)
+    return avail
+
+
+def cpu_baseline(hb, runs: int):
+    """The reference's kronmult_omp (built from its header, its own flags) on all host threads, full batch."""
+    import oracle
+    threads = cpu_threads()
+    kind = "reference"
+    if oracle.have_ref("shipped"):
+        runner = oracle.RefRunner(hb, "shipped")
+        call = lambda: runner.run(threads)
+        reset = runner.reset
+    else:  # the restatement, same arithmetic and threading scheme
+        kind = "port"
+        out = hb.output_pool_init()
+        call = lambda: oracle.kronmult_omp(hb, threads, out_pool=out)
+        reset = lambda: None
+    times = []
+    for r in range(runs + 1):
+        reset()
+        t0 = time.perf_counter()
+        call()
+        t1 = time.perf_counter()
+        if r > 0:
+            times.append(t1 - t0)
+        log(f"[cpu] run {r} {'(warm-up) ' if r == 0 else ''}{(t1 - t0) * 1e3:.1f} ms on {threads} threads")
+    med = statistics.median(times)
+    return {"value": round(hb.flops() / med / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": kind,
+            "sample": (f"full {hb.name} batch ({hb.nb} entries, {hb.flops() / 1e9:.3f} GFLOP), private input "
+                       f"copies, 1 warm-up + {runs} runs, median {med * 1e3:.1f} ms, min {min(times) * 1e3:.1f} ms")}
+
+
+def load_traffic(key: str):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    entry = table.get(key)
+    return None if entry is None else entry.get("bytes_per_launch")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        log("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
+            "(the operator would shard by output rows with no exchange; see DESIGN.md).")
+        return 2
+
+    import numpy as np
+    import torch
+
+    from kronmult993_amd import api, workloads as W
+    from kronmult993_amd.device import DeviceBatch
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    np_dtype = np.float64 if args.dtype == "f64" else np.float32
+    hb = W.by_name(args.config).astype(np_dtype)
+    log(f"[bench] {hb.name} {args.dtype}: d={hb.d} n={hb.n} nb={hb.nb} kernel="
+        f"{api.kernel_name(hb.d, hb.n, hb.dtype.itemsize, hb.nb)} on {torch.cuda.get_device_name(dev)}")
+    db = DeviceBatch(hb, device=str(dev))
+    stream = torch.cuda.current_stream(dev)
+
+    for _ in range(args.warmup):
+        db.run(sync=False, stream=stream)
+    torch.cuda.synchronize(dev)
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize(dev)
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    for i in range(args.steps):
+        starts[i].record(stream)
+        db.run(sync=False, stream=stream)
+        ends[i].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - wall0
+    total_ms = t_start.elapsed_time(t_end)
+    kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    ms_per_step = total_ms / args.steps
+    k_avg = statistics.mean(kernel_ms)
+    log(f"[bench] {args.steps} steps: {ms_per_step * 1e3:.2f} us/step (events), kernel avg {k_avg * 1e3:.2f} us, "
+        f"min {min(kernel_ms) * 1e3:.2f} us, wall {wall * 1e3 / args.steps:.2f} us/step")
+
+    flops = hb.flops()
+    contract = hb.contract_bytes()
+    gflops = flops / (ms_per_step * 1e-3) / 1e9
+    achieved_gbs = contract / (k_avg * 1e-3) / 1e9
+    peak_tf = FP64_PEAK_TFLOPS if args.dtype == "f64" else FP32_PEAK_TFLOPS
+    achieved_tf = flops / (k_avg * 1e-3) / 1e12
+    kernel = api.kernel_name(hb.d, hb.n, hb.dtype.itemsize, hb.nb)
+    result = {
+        "metric": "batched kron-matvec GFLOP/s & %roofline, d=6 n=4 FP64 vs FP32-split at 1e-10",
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic (seeded counter-based generator, seed 0)",
+        "config": {"workload": f"{hb.name}: {WORKLOADS[args.config]}", "d": hb.d, "n": hb.n,
+                   "matrix_stride": hb.stride, "nb_batch": hb.nb, "distinct_outputs": hb.out_count,
+                   "kernel": kernel},
+        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic(f"{args.config}_{args.dtype}")},
+        "roofline_compute": {"bound": "fp64" if args.dtype == "f64" else "fp32", "achieved": round(achieved_tf, 3),
+                             "peak": peak_tf, "unit": "TFLOP/s", "frac": round(achieved_tf / peak_tf, 4)},
+        "kernel_us": {"avg": round(k_avg * 1e3, 3), "min": round(min(kernel_ms) * 1e3, 3),
+                      "max": round(max(kernel_ms) * 1e3, 3)},
+        "contract_bytes_per_launch": contract,
+        "flops_per_launch": flops,
+    }
+    if args.cpu_baseline == "auto":
+        result["cpu_baseline"] = cpu_baseline(hb, args.cpu_runs)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

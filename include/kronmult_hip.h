@@ -1,0 +1,63 @@
+/*
+ * kronmult_hip.h -- C ABI of libkronmult_hip.so, the MI355X (gfx950) batched Kronecker mat-vec.
+ *
+ *   output_batched[k] += kron(A_{k,0}, ..., A_{k,d-1}) · input_batched[k],   0 <= k < nb_batch
+ *
+ * where A_{k,t} = matrix_list_batched[k*d + t] is an n x n column-major matrix with leading dimension
+ * `matrix_stride` (element (r, c) at A[r + c*matrix_stride]), A_{k,0} owns the slowest index of the n^d
+ * vectors, and output pointers may alias (accumulation is thread-safe).
+ *
+ * Every pointer, including the pointer arrays themselves, must be device-accessible (hipMalloc'ed or
+ * managed). Inputs and workspaces are never written by this library (the reference allows both to be
+ * clobbered); `workspace_batched` is read only as the address of per-entry scratch for vectors larger
+ * than the on-chip path takes (n^d * sizeof(T) > 64 KiB), and may be NULL otherwise.
+ *
+ * Return values are hipError_t codes cast to int (0 == hipSuccess). Invalid sizes (d < 1, n < 1,
+ * nb_batch < 0, matrix_stride < n, n^d >= 2^31) or NULL pointer arrays with nb_batch > 0 give
+ * hipErrorInvalidValue (1); nb_batch == 0 returns hipSuccess without launching anything.
+ */
+#ifndef KRONMULT_HIP_H
+#define KRONMULT_HIP_H
+
+#ifdef __cplusplus
+extern "C"
+{
+#endif
+
+/*
+ * Drop-in for kronmult_batched<double> / <float> (reference: kronmult_gpu/kronmult.cuh:28-32, specialised
+ * in kronmult_gpu/kronmult.cu:202-224). Launches on the null stream of the current device and returns
+ * after the device has finished (kronmult.cu:196 returns cudaDeviceSynchronize()).
+ */
+int kronmult_batched_f64(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                         int matrix_stride, double **input_batched, double **output_batched,
+                         double **workspace_batched, int nb_batch);
+int kronmult_batched_f32(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
+                         int matrix_stride, float **input_batched, float **output_batched,
+                         float **workspace_batched, int nb_batch);
+
+/*
+ * Stream-ordered variants of the two entry points above (no counterpart in the reference, which always
+ * uses the legacy default stream and synchronises): same arguments plus a hipStream_t passed as void*
+ * (NULL = null stream). They return after the launch; errors of the launch itself are reported, errors
+ * of the kernel surface at the caller's next synchronisation.
+ */
+int kronmult_batched_f64_async(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                               int matrix_stride, double **input_batched, double **output_batched,
+                               double **workspace_batched, int nb_batch, void *stream);
+int kronmult_batched_f32_async(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
+                               int matrix_stride, float **input_batched, float **output_batched,
+                               float **workspace_batched, int nb_batch, void *stream);
+
+/*
+ * Library version string and the name of the kernel the dispatcher would pick for (d, n, element size in
+ * bytes, nb_batch); both point to static storage. No reference counterpart (diagnostics only).
+ */
+char const *kronmult_hip_version(void);
+char const *kronmult_hip_kernel_name(int matrix_count, int matrix_size, int element_size, int nb_batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KRONMULT_HIP_H */

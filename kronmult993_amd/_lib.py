@@ -1,0 +1,59 @@
+"""ctypes loader for libkronmult_hip.so (the C ABI declared in include/kronmult_hip.h).
+
+The library is built in-tree by kronmult993_amd/csrc/Makefile. There is no fallback: if it is missing the
+import of any compute entry point raises, so nothing can silently run on a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkronmult_hip.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(HERE), "include")
+
+_lib = None
+_lock = threading.Lock()
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+
+# name -> (restype, argtypes); the order of arguments is the one of include/kronmult_hip.h
+SIGNATURES = {
+    "kronmult_batched_f64": (_I, [_I, _I, _P, _I, _P, _P, _P, _I]),
+    "kronmult_batched_f32": (_I, [_I, _I, _P, _I, _P, _P, _P, _I]),
+    "kronmult_batched_f64_async": (_I, [_I, _I, _P, _I, _P, _P, _P, _I, _P]),
+    "kronmult_batched_f32_async": (_I, [_I, _I, _P, _I, _P, _P, _P, _I, _P]),
+    "kronmult_hip_version": (ctypes.c_char_p, []),
+    "kronmult_hip_kernel_name": (ctypes.c_char_p, [_I, _I, _I, _I]),
+}
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def build() -> None:
+    """Compile the library with hipcc for gfx950 (in-tree)."""
+    import subprocess
+    subprocess.run(["make", "-C", os.path.join(HERE, "csrc"), "-B"], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises LibraryMissing if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise LibraryMissing(f"{LIB_PATH} is not built: run `make -C kronmult993_amd/csrc` "
+                                     "(or __graft_entry__.build()) first")
+            handle = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                f = getattr(handle, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = handle
+    return _lib

@@ -1,0 +1,87 @@
+"""Host-side mirror of the reference's `kronmult_batched` interface, over the C ABI.
+
+`kronmult_batched(matrix_count, matrix_size, matrix_list_batched, matrix_stride, input_batched,
+output_batched, workspace_batched, nb_batch)` takes the same arguments, in the same order and with the same
+meaning, as the reference (kronmult_gpu/kronmult.cuh:28-32): pointer arrays of device addresses (here a
+torch int64 tensor on the device, or an integer address of such an array), column-major factors with
+leading dimension `matrix_stride`, outputs accumulated and possibly aliased. The element type, a template
+parameter in C++, is the keyword `dtype`.
+
+Errors: the reference returns the cudaError of cudaDeviceSynchronize and its harness throws on failure
+(tests/utils/utils_gpu.h:11-17). Here a non-zero HIP status raises `KronmultError` carrying the code.
+"""
+from __future__ import annotations
+
+from typing import Any, Optional
+
+from ._lib import lib
+
+HIP_SUCCESS = 0
+HIP_ERROR_INVALID_VALUE = 1
+
+
+class KronmultError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: hipError {code}")
+        self.code = code
+
+
+def _addr(x: Any) -> Optional[int]:
+    """Device address of a pointer array: torch tensor -> data_ptr(), int -> itself, None -> NULL."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    ptr = getattr(x, "data_ptr", None)
+    if ptr is not None:
+        return int(ptr())
+    raise TypeError(f"cannot take the address of {type(x)!r}")
+
+
+def _suffix(dtype: Any) -> str:
+    s = str(dtype).replace("torch.", "").replace("numpy.", "")
+    if s in ("float64", "double", "f64", "<class 'numpy.float64'>"):
+        return "f64"
+    if s in ("float32", "float", "f32", "<class 'numpy.float32'>"):
+        return "f32"
+    raise TypeError(f"kronmult_batched is provided for float32 and float64 only, not {dtype}")
+
+
+def _stream_handle(stream: Any) -> Optional[int]:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream or None
+    h = getattr(stream, "cuda_stream", None)  # torch.cuda.Stream
+    if h is not None:
+        return int(h) or None
+    raise TypeError(f"not a stream: {type(stream)!r}")
+
+
+def kronmult_batched(matrix_count: int, matrix_size: int, matrix_list_batched, matrix_stride: int,
+                     input_batched, output_batched, workspace_batched, nb_batch: int, *, dtype) -> None:
+    """Synchronous drop-in call (returns once the device is idle, like the reference)."""
+    f = getattr(lib(), "kronmult_batched_" + _suffix(dtype))
+    rc = f(matrix_count, matrix_size, _addr(matrix_list_batched), matrix_stride, _addr(input_batched),
+           _addr(output_batched), _addr(workspace_batched), nb_batch)
+    if rc != HIP_SUCCESS:
+        raise KronmultError(rc, "kronmult_batched")
+
+
+def kronmult_batched_async(matrix_count: int, matrix_size: int, matrix_list_batched, matrix_stride: int,
+                           input_batched, output_batched, workspace_batched, nb_batch: int, *, dtype,
+                           stream=None) -> None:
+    """Stream-ordered call; `stream` is a torch.cuda.Stream, a raw hipStream_t address, or None."""
+    f = getattr(lib(), "kronmult_batched_" + _suffix(dtype) + "_async")
+    rc = f(matrix_count, matrix_size, _addr(matrix_list_batched), matrix_stride, _addr(input_batched),
+           _addr(output_batched), _addr(workspace_batched), nb_batch, _stream_handle(stream))
+    if rc != HIP_SUCCESS:
+        raise KronmultError(rc, "kronmult_batched_async")
+
+
+def kernel_name(matrix_count: int, matrix_size: int, element_size: int, nb_batch: int) -> str:
+    return lib().kronmult_hip_kernel_name(matrix_count, matrix_size, element_size, nb_batch).decode()
+
+
+def version() -> str:
+    return lib().kronmult_hip_version().decode()

@@ -1,0 +1,76 @@
+"""A `HostBatch` placed in device memory, with the device pointer arrays the C ABI takes.
+
+torch is used only to own device memory and to name streams; every computation goes through
+libkronmult_hip.so.
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+
+from . import api
+from .workloads import HostBatch
+
+_LDS_BYTES = 64 * 1024  # vectors above this size run from the per-entry workspace
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class DeviceBatch:
+    """Device pools + pointer arrays for one HostBatch.
+
+    workspace: "auto" allocates per-entry workspaces only when the library needs them (n^d * itemsize
+    above the on-chip size), "per_entry" always, "none" never.
+    """
+
+    def __init__(self, hb: HostBatch, device: str = "cuda", workspace: str = "auto"):
+        torch = _torch()
+        self.hb = hb
+        self.device = torch.device(device)
+        self.d, self.n, self.stride, self.nb, self.size = hb.d, hb.n, hb.stride, hb.nb, hb.size
+        self.np_dtype = hb.dtype
+        self.itemsize = hb.dtype.itemsize
+        self.mat = torch.from_numpy(np.ascontiguousarray(hb.mat_pool)).to(self.device)
+        self.inp = torch.from_numpy(np.ascontiguousarray(hb.in_pool)).to(self.device)
+        self._out_init = torch.from_numpy(hb.output_pool_init()).to(self.device)
+        self.out = self._out_init.clone()
+        self.mat_ptrs = self._ptrs(self.mat, hb.mat_off)
+        self.in_ptrs = self._ptrs(self.inp, hb.in_off)
+        self.out_ptrs = self._ptrs(self.out, hb.out_off)
+        need_ws = workspace == "per_entry" or (workspace == "auto" and self.size * self.itemsize > _LDS_BYTES)
+        if need_ws:
+            self.ws = torch.zeros(self.nb * self.size, dtype=self.out.dtype, device=self.device)
+            self.ws_ptrs = self._ptrs(self.ws, np.arange(self.nb, dtype=np.int64) * self.size)
+        else:
+            self.ws = None
+            self.ws_ptrs = None
+
+    def _ptrs(self, t, off: np.ndarray):
+        torch = _torch()
+        addr = np.int64(t.data_ptr()) + off.astype(np.int64) * np.int64(self.itemsize)
+        return torch.from_numpy(addr).to(self.device)
+
+    @property
+    def dtype_name(self) -> str:
+        return "f64" if self.itemsize == 8 else "f32"
+
+    def reset_output(self) -> None:
+        self.out.copy_(self._out_init)
+
+    def run(self, sync: bool = True, stream=None) -> None:
+        """One call of the library over the whole batch."""
+        args = (self.d, self.n, self.mat_ptrs, self.stride, self.in_ptrs, self.out_ptrs, self.ws_ptrs, self.nb)
+        if sync:
+            api.kronmult_batched(*args, dtype=self.dtype_name)
+        else:
+            api.kronmult_batched_async(*args, dtype=self.dtype_name, stream=stream)
+
+    def output(self) -> np.ndarray:
+        return self.out.detach().cpu().numpy()
+
+    def inputs(self) -> np.ndarray:
+        return self.inp.detach().cpu().numpy()

@@ -1,0 +1,131 @@
+"""CPU tests of the host side: generator, workload builders, the C-ABI library's exports and its argument
+checks (which return before touching a device)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from kronmult993_amd import _lib, gen, workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ---- generator --------------------------------------------------------------------------------------
+
+def test_generator_is_deterministic_and_counter_based():
+    a = gen.uniform(0, 1, 1000)
+    assert np.array_equal(a, gen.uniform(0, 1, 1000))
+    assert np.array_equal(a[500:], gen.uniform(0, 1, 500, offset=500))
+    assert not np.array_equal(a, gen.uniform(0, 2, 1000))
+    assert not np.array_equal(a, gen.uniform(1, 1, 1000))
+
+
+def test_splitmix64_known_answer():
+    """SplitMix64's published finaliser: mix64(0x9E3779B97F4A7C15) (the first output of the generator seeded
+    with 0) is 0xE220A8397B1DCDAF."""
+    z = gen._mix64(np.array([0x9E3779B97F4A7C15], dtype=np.uint64))
+    assert int(z[0]) == 0xE220A8397B1DCDAF
+
+
+def test_generator_distributions():
+    u = gen.uniform(3, 4, 200000)
+    assert u.min() >= -1.0 and u.max() < 1.0
+    assert abs(u.mean()) < 0.01 and abs(u.var() - 1.0 / 3.0) < 0.01
+    z = gen.normal(3, 5, 200000)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1.0) < 0.01
+
+
+# ---- workloads ----------------------------------------------------------------------------------------
+
+def test_c2_digit_map():
+    b = W.c2()
+    assert (b.d, b.n, b.stride, b.nb, b.out_count) == (6, 4, 64, 8192, 128)
+    moff = b.mat_off.reshape(b.nb, b.d)
+    for k in (0, 1, 63, 64, 1000, 8191):
+        i, j = divmod(k, 64)
+        assert b.in_off[k] == j * 4096 and b.out_index[k] == i
+        for t in range(6):
+            r, c = (i >> (4 * t)) & 15, (j >> (4 * t)) & 15
+            assert moff[k, t] == t * 4096 + 4 * r + 4 * c * 64
+    assert np.all(moff[:, 2:] % 4096 == 0)  # digits 2..5 of ids < 256 are zero
+
+
+def test_config_shapes():
+    c1 = W.c1()
+    assert (c1.d, c1.n, c1.nb, c1.out_count) == (3, 4, 4096, 512)
+    assert np.array_equal(np.bincount(c1.out_index), np.full(512, 8))
+    c4 = W.c4(nb=64)
+    assert (c4.d, c4.n, c4.size) == (4, 8, 4096)
+    scale = np.abs(c4.mat_pool.reshape(-1, 64)).max(axis=1)
+    assert scale.min() < 0.1 and scale.max() > 10.0
+    c5 = W.c5(nb=1024)
+    assert np.array_equal(c5.mat_off[:3], c5.mat_off[256 * 3: 256 * 3 + 3])
+    assert c5.grad_out_pool.shape == (256 * 512,)
+    groups = W.c3()
+    assert [(g.d, g.n, g.nb) for g in groups] == [(2, 8, 20000), (3, 6, 6000), (4, 4, 3000), (5, 3, 1500),
+                                                (6, 2, 800)]
+    assert all(np.array_equal(np.bincount(g.out_index), np.full(g.nb // 4, 4)) for g in groups)
+
+
+def test_reference_batch_size_model():
+    """tests/utils/batch_size.h:8-21 for the reference harness cases."""
+    assert W.compute_batch_size(4, 1, 2, 5) == 4
+    assert W.compute_batch_size(4, 2, 4, 5) == 64
+    assert W.compute_batch_size(6, 3, 6, 5) == 384
+    assert W.compute_batch_size(8, 6, 7, 5) == 896
+    assert W.compute_batch_size(8, 6, 9, 5) == 3903  # the published "large"/"realistic" batch
+
+
+def test_reference_alias_map():
+    """Entry i >= distinct goes to output (i*distinct)/N (tests/utils/utils_cpu.h:79-83)."""
+    b = W.reference_case(4, 2, 4)
+    assert b.nb == 64 and b.out_count == 5
+    expect = [i if i < 5 else (i * 5) // 64 for i in range(64)]
+    assert list(b.out_index) == expect
+
+
+def test_flop_and_byte_model():
+    assert W.flops_per_entry(6, 4) == 200704
+    assert W.bytes_per_entry(6, 4, 8) == 99136
+    assert W.flops_per_entry(6, 8) == 25427968
+    assert W.c2().flops() == 8192 * 200704
+
+
+# ---- C ABI ----------------------------------------------------------------------------------------------
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "kronmult_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(kronmult_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    handle = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared_functions()
+    assert "kronmult_batched_f64" in names and "kronmult_batched_f32" in names
+    for name in names:
+        assert hasattr(handle, name), name
+    assert set(names) <= set(_lib.SIGNATURES), "ctypes signatures out of date"
+
+
+def test_invalid_arguments_are_rejected_before_any_device_call():
+    lib = _lib.lib()
+    dummy = ctypes.c_void_p(16)
+    for d, n, stride, nb in ((0, 4, 4, 1), (2, 0, 4, 1), (2, 4, 3, 1), (2, 4, 4, -1), (40, 2, 2, 1)):
+        assert lib.kronmult_batched_f64(d, n, dummy, stride, dummy, dummy, dummy, nb) == 1
+        assert lib.kronmult_batched_f32_async(d, n, dummy, stride, dummy, dummy, dummy, nb, None) == 1
+    # nb_batch == 0: success, nothing launched
+    assert lib.kronmult_batched_f64(3, 4, None, 4, None, None, None, 0) == 0
+    assert lib.kronmult_batched_f64_async(3, 4, None, 4, None, None, None, 0, None) == 0
+    # null pointer arrays with work to do
+    assert lib.kronmult_batched_f64_async(3, 4, None, 4, dummy, dummy, None, 5, None) == 1
+
+
+def test_kernel_selection():
+    from kronmult993_amd import api
+    assert api.kernel_name(6, 4, 8, 8192).startswith("fiber_lds")
+    assert api.kernel_name(6, 8, 8, 3903).startswith("fiber_global")
+    assert api.kernel_name(2, 100, 8, 1) == "unsupported"
+    assert "gfx950" in api.version()

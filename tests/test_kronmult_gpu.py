@@ -1,0 +1,160 @@
+"""GPU parity of libkronmult_hip.so against the oracle, through the C ABI.
+
+Bar (FP64 and FP32 alike):
+  * an entry's contribution is bit-identical to the oracle's fused-multiply-add sweep, so wherever every
+    output receives one contribution the result is compared with np.array_equal;
+  * with aliased outputs only the order of the atomic additions differs: the error must stay within the
+    componentwise bound (E_o + 2) * u * (|K| |x| summed over contributions), and the normwise relative error
+    within 1e-13 (FP64) / 1e-5 (FP32).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import accumulation_bound, normwise
+from kronmult993_amd import api, workloads as W
+from kronmult993_amd.device import DeviceBatch
+
+pytestmark = pytest.mark.gpu
+
+EPS = {np.dtype(np.float64): np.finfo(np.float64).eps / 2, np.dtype(np.float32): np.finfo(np.float32).eps / 2}
+NORM_TOL = {np.dtype(np.float64): 1e-13, np.dtype(np.float32): 1e-5}
+
+
+def _unaliased(b):
+    """Same entries, each into its own zero-initialised output."""
+    return W.HostBatch(b.name + "_unaliased", b.d, b.n, b.stride, b.mat_pool, b.mat_off, b.in_pool, b.in_off,
+                       b.nb, np.arange(b.nb, dtype=np.int64))
+
+
+def _run(b, gpu, **kw):
+    print(f"[case] {b.name} d={b.d} n={b.n} nb={b.nb} stride={b.stride} dtype={b.dtype} "
+          f"kernel={api.kernel_name(b.d, b.n, b.dtype.itemsize, b.nb)}", flush=True)
+    db = DeviceBatch(b, device=str(gpu), **kw)
+    inputs_before = db.inp.clone()
+    db.run()
+    out = db.output()
+    assert bool((db.inp == inputs_before).all()), "inputs must be left unmodified"
+    return out
+
+
+def _check_aliased(b, out):
+    ref = oracle.kronmult(b)
+    bound = accumulation_bound(b, EPS[b.dtype])
+    err = np.abs(out.astype(np.float64) - ref.astype(np.float64))
+    worst = float(np.max(err - bound))
+    assert worst <= 0.0, f"{b.name}: componentwise bound exceeded by {worst:.3e}"
+    assert normwise(out, ref) <= NORM_TOL[b.dtype]
+
+
+def _check_bitexact(b, out):
+    ref = oracle.kronmult(b)
+    if not np.array_equal(out, ref):
+        bad = np.flatnonzero(out != ref)
+        pytest.fail(f"{b.name}: {bad.size} elements differ, first at {bad[0]}: {out[bad[0]]!r} vs {ref[bad[0]]!r}, "
+                    f"normwise {normwise(out, ref):.3e}")
+
+
+DTYPES = [np.float64, np.float32]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_c1_full(gpu, dtype):
+    b = W.c1().astype(dtype)
+    _check_aliased(b, _run(b, gpu))
+    u = _unaliased(b)
+    _check_bitexact(u, _run(u, gpu))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_c2_asgard_full(gpu, dtype):
+    b = W.c2().astype(dtype)
+    _check_aliased(b, _run(b, gpu))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_c2_entries_bitexact(gpu, dtype):
+    u = _unaliased(W.c2(rows=3, cols=64).astype(dtype))
+    _check_bitexact(u, _run(u, gpu))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("group", range(5))
+def test_c3_groups(gpu, dtype, group):
+    b = W.c3()[group].astype(dtype)
+    _check_aliased(b, _run(b, gpu))
+    u = _unaliased(b.subset(min(b.nb, 2000)))
+    _check_bitexact(u, _run(u, gpu))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_c4_range(gpu, dtype):
+    b = W.c4(nb=4096).astype(dtype)
+    _check_aliased(b, _run(b, gpu))
+    u = _unaliased(W.c4(nb=256).astype(dtype))
+    _check_bitexact(u, _run(u, gpu))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_c5_forward(gpu, dtype):
+    b = W.c5().astype(dtype)
+    _check_aliased(b, _run(b, gpu))
+
+
+@pytest.mark.parametrize("deg,dim,lvl", [(4, 1, 2), (4, 2, 4), (6, 3, 6)])
+def test_reference_harness_cases(gpu, deg, dim, lvl):
+    """The reference's own test cases (tests/kronmult_test_gpu.cpp:148-149 and the bench's medium case):
+    stride 67, N(0,1) data, non-zero outputs aliased by tests/utils/utils_cpu.h:79-83."""
+    b = W.reference_case(deg, dim, lvl)
+    _check_aliased(b, _run(b, gpu))
+
+
+@pytest.mark.parametrize("d,n,nb,seed", [
+    (1, 1, 7, 1), (1, 4, 33, 2), (2, 3, 50, 3), (2, 7, 65, 4), (3, 5, 40, 5), (4, 6, 9, 6), (5, 2, 100, 7),
+    (6, 3, 20, 8), (2, 9, 30, 9), (3, 10, 12, 10), (2, 16, 8, 11), (1, 64, 5, 12), (7, 2, 33, 13),
+    (8, 3, 3, 14)])
+def test_random_shapes(gpu, d, n, nb, seed):
+    for dtype in DTYPES:
+        b = W.random_case(seed, d, n, nb, shared_factors=max(1, nb // 4)).astype(dtype)
+        _check_aliased(b, _run(b, gpu))
+        u = _unaliased(b)
+        _check_bitexact(u, _run(u, gpu))
+
+
+@pytest.mark.parametrize("d,n,nb", [(3, 21, 6), (6, 7, 3), (4, 12, 4)])
+def test_large_vectors_use_workspace(gpu, d, n, nb):
+    """n^d * 8 bytes above the on-chip size: the sweep runs from the caller's workspace."""
+    b = W.random_case(40 + d, d, n, nb)
+    assert api.kernel_name(d, n, 8, nb).startswith("fiber_global")
+    _check_aliased(b, _run(b, gpu))
+    u = _unaliased(b)
+    _check_bitexact(u, _run(u, gpu))
+
+
+def test_empty_batch_and_invalid_arguments(gpu):
+    b = W.random_case(50, 3, 4, 8)
+    db = DeviceBatch(b, device=str(gpu))
+    before = db.output()
+    api.kronmult_batched(3, 4, db.mat_ptrs, b.stride, db.in_ptrs, db.out_ptrs, None, 0, dtype="f64")
+    assert np.array_equal(db.output(), before)
+    for args in ((0, 4, b.stride), (3, 0, b.stride), (3, 4, 3)):
+        with pytest.raises(api.KronmultError) as e:
+            api.kronmult_batched(args[0], args[1], db.mat_ptrs, args[2], db.in_ptrs, db.out_ptrs, None, 8,
+                                 dtype="f64")
+        assert e.value.code == api.HIP_ERROR_INVALID_VALUE
+    with pytest.raises(api.KronmultError):
+        api.kronmult_batched(3, 4, None, b.stride, db.in_ptrs, db.out_ptrs, None, 8, dtype="f64")
+    assert np.array_equal(db.output(), before)
+
+
+def test_async_on_a_stream_and_repeat_accumulates(gpu):
+    import torch
+    b = W.random_case(51, 4, 4, 64)
+    db = DeviceBatch(b, device=str(gpu))
+    s = torch.cuda.Stream(device=gpu)
+    db.run(sync=False, stream=s)
+    db.run(sync=False, stream=s)
+    s.synchronize()
+    once = oracle.kronmult(b)
+    twice = oracle.kronmult(b, out_pool=once.copy())
+    assert normwise(db.output(), twice) <= 1e-13

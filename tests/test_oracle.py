@@ -1,0 +1,275 @@
+"""Pins the CPU oracle (oracle/) before anything is checked against it.
+
+Each check is chosen so that a plausible mistake in the oracle (a factor transposed, the factor order
+reversed, a dropped term, a wrong stride) fails at least one of them:
+  * bit-exact agreement with the reference's own kronmult_omp and kronmult_naive, compiled from
+    /root/reference (oracle/_ref), in both arithmetic readings of `dotprod += x*a`;
+  * np.kron brute force (A_0 the slowest index, tests/utils/kronmult_naive.h:99-101);
+  * closed forms: identity, permutation, diagonal, rank-1 factors, d=1 GEMV, d=2 vec(B X A^T);
+  * invariants: linearity in x, multilinearity in each factor, the mixed-product property, aliasing;
+  * the adjoint against central finite differences and the dense Kronecker gradient.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from kronmult993_amd import gen, workloads as W
+from helpers import factors, kron_dense, normwise, numpy_kronmult
+
+needs_ref = pytest.mark.skipif(not oracle.have_ref("fused"),
+                               reason="oracle/_ref not built (the reference tree was absent at build time)")
+
+CASES = [
+    lambda: W.c1(nb=512),
+    lambda: W.c2(rows=2, cols=8),
+    lambda: W.reference_case(4, 1, 2),
+    lambda: W.reference_case(4, 2, 4),
+    lambda: W.reference_case(6, 3, 6),
+    lambda: W.reference_case(3, 4, 3),
+    lambda: W.c5(nb=64),
+    lambda: W.c4(nb=16),
+    lambda: W.random_case(1, 5, 3, 40),
+    lambda: W.random_case(2, 2, 7, 50),
+    lambda: W.random_case(3, 1, 9, 30),
+    lambda: W.random_case(4, 3, 5, 20, shared_factors=4),
+] + [lambda g=g: g for g in W.c3(scale=0.01)]
+
+
+@needs_ref
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_oracle_bitexact_vs_reference(case, dtype):
+    b = CASES[case]().astype(dtype)
+    for fused, variant in ((True, "fused"), (False, "unfused")):
+        mine = oracle.kronmult(b, fused=fused)
+        ref = oracle.ref_kronmult(b, nthreads=1, variant=variant)
+        assert np.array_equal(mine, ref), (b.name, variant, normwise(mine, ref))
+
+
+@needs_ref
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_oracle_close_to_reference_as_shipped(case):
+    """The reference built with its own flags mixes both readings; it stays within a few ulps."""
+    b = CASES[case]()
+    mine = oracle.kronmult(b)
+    ref = oracle.ref_kronmult(b, nthreads=1, variant="shipped")
+    assert normwise(mine, ref) < 64 * np.finfo(np.float64).eps
+
+
+@needs_ref
+@pytest.mark.parametrize("case", [0, 2, 3, 4, 5, 6, 8, 9, 10, 11])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_dense_oracle_bitexact_vs_reference_naive(case, dtype):
+    b = CASES[case]().astype(dtype)
+    if b.size > 1024:
+        pytest.skip("dense oracle kept to n^d <= 1024 here")
+    b = b.subset(min(b.nb, 64))
+    for fused, variant in ((True, "fused"), (False, "unfused")):
+        assert np.array_equal(oracle.dense(b, fused=fused), oracle.ref_naive(b, variant=variant))
+
+
+@pytest.mark.parametrize("d,n,nb,seed", [(1, 4, 8, 1), (2, 3, 10, 2), (3, 4, 6, 3), (4, 2, 9, 4), (2, 5, 7, 5),
+                                         (3, 3, 12, 6), (1, 1, 3, 7), (4, 3, 4, 8)])
+def test_oracle_vs_numpy_kron(d, n, nb, seed):
+    b = W.random_case(seed, d, n, nb)
+    ref = numpy_kronmult(b)
+    assert normwise(oracle.kronmult(b), ref) < 1e-14
+    assert normwise(oracle.kronmult(b, fused=False), ref) < 1e-14
+    assert normwise(oracle.dense(b), ref) < 1e-14
+
+
+def _with_factors(b, mats_per_entry):
+    """Replace the factors by explicit per-entry matrices (list over entries of lists of n x n arrays)."""
+    n, d = b.n, b.d
+    pool = np.zeros(b.nb * d * n * n, dtype=b.dtype)
+    for k, ms in enumerate(mats_per_entry):
+        for t, m in enumerate(ms):
+            pool[(k * d + t) * n * n:(k * d + t + 1) * n * n] = np.asarray(m, dtype=b.dtype).reshape(-1, order="F")
+    return W.HostBatch(b.name, d, n, n, pool, np.arange(b.nb * d, dtype=np.int64) * n * n, b.in_pool, b.in_off,
+                       b.out_count, b.out_index, out_init=None)
+
+
+def _inputs(b, k):
+    return b.in_pool[int(b.in_off[k]): int(b.in_off[k]) + b.size]
+
+
+def _outputs(out, b, o):
+    return out[o * b.size:(o + 1) * b.size]
+
+
+def test_closed_form_identity():
+    b = W.random_case(11, 4, 3, 5, distinct=5)
+    b = _with_factors(b, [[np.eye(3)] * 4 for _ in range(b.nb)])
+    out = oracle.kronmult(b)
+    for k in range(b.nb):
+        assert np.array_equal(_outputs(out, b, int(b.out_index[k])), _inputs(b, k))
+
+
+def test_closed_form_permutation():
+    rng_perm = [np.argsort(gen.raw64(12, t, 4)) for t in range(3)]
+    P = [np.eye(4)[p] for p in rng_perm]  # P[t] @ v == v[p]
+    b = W.random_case(12, 3, 4, 4, distinct=4)
+    b = _with_factors(b, [P for _ in range(b.nb)])
+    out = oracle.kronmult(b)
+    for k in range(b.nb):
+        x = _inputs(b, k).reshape(4, 4, 4)  # index (i0, i1, i2), i0 slowest
+        y = x[rng_perm[0]][:, rng_perm[1]][:, :, rng_perm[2]]
+        assert np.array_equal(_outputs(out, b, int(b.out_index[k])), y.reshape(-1))
+
+
+def test_closed_form_diagonal():
+    b = W.random_case(13, 3, 5, 3, distinct=3)
+    diags = [[gen.uniform(13, 40 + 3 * k + t, 5) for t in range(3)] for k in range(3)]
+    b = _with_factors(b, [[np.diag(v) for v in dk] for dk in diags])
+    out = oracle.kronmult(b)
+    for k in range(3):
+        scale = kron_dense([v[None, :] for v in diags[k]]).reshape(-1)
+        assert normwise(_outputs(out, b, int(b.out_index[k])), scale * _inputs(b, k)) < 1e-15
+
+
+def test_closed_form_rank_one():
+    d, n = 3, 4
+    b = W.random_case(14, d, n, 2, distinct=2)
+    us = [gen.uniform(14, 50 + t, n) for t in range(d)]
+    vs = [gen.uniform(14, 60 + t, n) for t in range(d)]
+    b = _with_factors(b, [[np.outer(u, v) for u, v in zip(us, vs)] for _ in range(2)])
+    out = oracle.kronmult(b)
+    U = kron_dense([u[:, None] for u in us]).reshape(-1)
+    V = kron_dense([v[:, None] for v in vs]).reshape(-1)
+    for k in range(2):
+        assert normwise(_outputs(out, b, int(b.out_index[k])), U * (V @ _inputs(b, k))) < 1e-14
+
+
+def test_closed_form_d1_gemv_and_d2_matrix_form():
+    b1 = W.random_case(15, 1, 7, 4, distinct=4)
+    b1 = W.HostBatch(**{**b1.__dict__, "out_init": None})
+    out = oracle.kronmult(b1)
+    for k in range(4):
+        A = factors(b1, k)[0]
+        assert normwise(_outputs(out, b1, int(b1.out_index[k])), A @ _inputs(b1, k)) < 1e-15
+    b2 = W.random_case(16, 2, 5, 4, distinct=4)
+    b2 = W.HostBatch(**{**b2.__dict__, "out_init": None})
+    out = oracle.kronmult(b2)
+    for k in range(4):
+        A, B = factors(b2, k)
+        X = _inputs(b2, k).reshape(5, 5, order="F")  # vec(X) column-major: index = i1 + 5*i0 -> X[i1, i0]
+        Y = B @ X @ A.T                               # (A ⊗ B) vec(X) = vec(B X A^T)
+        assert normwise(_outputs(out, b2, int(b2.out_index[k])), Y.reshape(-1, order="F")) < 1e-14
+
+
+def test_linearity_and_multilinearity():
+    b = W.random_case(17, 3, 4, 6, distinct=6)
+    b = W.HostBatch(**{**b.__dict__, "out_init": None})
+    base = oracle.kronmult(b)
+    b2 = W.HostBatch(**{**b.__dict__, "in_pool": 2.0 * b.in_pool})
+    assert np.array_equal(oracle.kronmult(b2) - b.output_pool_init(), 2.0 * (base - b.output_pool_init()))
+    pool = b.mat_pool.copy()
+    for k in range(b.nb):  # scale factor 1 of every entry by 3
+        off = int(b.mat_off[k * b.d + 1])
+        for c in range(b.n):
+            pool[off + c * b.stride: off + c * b.stride + b.n] *= 3.0
+    b3 = W.HostBatch(**{**b.__dict__, "mat_pool": pool})
+    assert normwise(oracle.kronmult(b3) - b.output_pool_init(), 3.0 * (base - b.output_pool_init())) < 1e-15
+
+
+def test_mixed_product():
+    """kron(A_t) · (kron(C_t) · x) == kron(A_t C_t) · x."""
+    d, n = 3, 3
+    A = [[gen.uniform(18, 70 + t, n * n).reshape(n, n) for t in range(d)]]
+    C = [[gen.uniform(18, 80 + t, n * n).reshape(n, n) for t in range(d)]]
+    b = W.random_case(18, d, n, 1, distinct=1)
+    bc = _with_factors(b, C)
+    y = oracle.kronmult(bc)
+    ba = _with_factors(W.HostBatch(**{**b.__dict__, "in_pool": y}), A)
+    lhs = oracle.kronmult(ba)
+    bac = _with_factors(b, [[a @ c for a, c in zip(A[0], C[0])]])
+    assert normwise(lhs, oracle.kronmult(bac)) < 1e-13
+
+
+def test_aliasing_equals_multiple():
+    """E identical entries into one output == E times the single contribution."""
+    one = W.random_case(19, 3, 4, 1, distinct=1)
+    many = W.HostBatch(one.name, one.d, one.n, one.stride, one.mat_pool, np.tile(one.mat_off, 8), one.in_pool,
+                       np.tile(one.in_off, 8), 1, np.zeros(8, dtype=np.int64))
+    single = oracle.kronmult(W.HostBatch(**{**one.__dict__, "out_init": None}))
+    assert normwise(oracle.kronmult(many), 8.0 * single) < 1e-15
+
+
+def _loss(b, g):
+    return float(np.dot(oracle.kronmult(b), g))
+
+
+@pytest.mark.parametrize("d,n,shared", [(3, 3, None), (2, 4, 2), (1, 5, None)])
+def test_adjoint_vs_finite_differences(d, n, shared):
+    b = W.random_case(20 + d, d, n, 6, distinct=3, shared_factors=shared)
+    b = W.HostBatch(**{**b.__dict__, "out_init": None})
+    g = b.grad_out_pool
+    gi, gm = oracle.adjoint(b, g)
+    h = 1e-6
+    for idx in range(0, b.in_pool.size, 7):
+        p, m = b.in_pool.copy(), b.in_pool.copy()
+        p[idx] += h
+        m[idx] -= h
+        fd = (_loss(W.HostBatch(**{**b.__dict__, "in_pool": p}), g) -
+              _loss(W.HostBatch(**{**b.__dict__, "in_pool": m}), g)) / (2 * h)
+        assert abs(fd - gi[idx]) <= 1e-7 * max(1.0, abs(fd))
+    for idx in range(0, b.mat_pool.size, 3):
+        if gm[idx] == 0.0 and not np.any(b.mat_off <= idx):
+            continue
+        p, m = b.mat_pool.copy(), b.mat_pool.copy()
+        p[idx] += h
+        m[idx] -= h
+        fd = (_loss(W.HostBatch(**{**b.__dict__, "mat_pool": p}), g) -
+              _loss(W.HostBatch(**{**b.__dict__, "mat_pool": m}), g)) / (2 * h)
+        assert abs(fd - gm[idx]) <= 1e-7 * max(1.0, abs(fd)), idx
+
+
+def test_adjoint_vs_dense_kronecker_gradient():
+    d, n = 3, 3
+    b = W.random_case(24, d, n, 4, distinct=2)
+    g = b.grad_out_pool
+    gi, gm = oracle.adjoint(b, g)
+    size = b.size
+    exp_gi = np.zeros_like(b.in_pool)
+    exp_gm = np.zeros_like(b.mat_pool)
+    for k in range(b.nb):
+        F = factors(b, k)
+        gk = g[int(b.out_off[k]): int(b.out_off[k]) + size]
+        xk = _inputs(b, k)
+        exp_gi[int(b.in_off[k]): int(b.in_off[k]) + size] += kron_dense(F).T @ gk
+        for t in range(d):
+            off = int(b.mat_off[k * d + t])
+            for r in range(n):
+                for c in range(n):
+                    E = np.zeros((n, n))
+                    E[r, c] = 1.0
+                    exp_gm[off + r + c * b.stride] += gk @ kron_dense(F[:t] + [E] + F[t + 1:]) @ xk
+    assert normwise(gi, exp_gi) < 1e-14
+    assert normwise(gm, exp_gm) < 1e-14
+
+
+def test_high_precision_oracle():
+    b = W.c4(nb=8)
+    hp = oracle.kronmult_hp(b)
+    assert normwise(oracle.kronmult(b), hp) < 1e-14
+    assert normwise(oracle.kronmult(b.astype(np.float32)).astype(np.float64), hp) > 1e-9  # f32 is visibly worse
+
+
+# ---- planted mistakes: the checks above must catch them ------------------------------------------------
+
+def test_planted_transposed_factor_is_caught():
+    b = W.random_case(25, 3, 4, 3)
+    pool = b.mat_pool.copy()
+    off = int(b.mat_off[1])  # entry 0, factor 1
+    M = factors(b, 0)[1]
+    for c in range(b.n):
+        pool[off + c * b.stride: off + c * b.stride + b.n] = M.T[:, c]
+    mutated = oracle.kronmult(W.HostBatch(**{**b.__dict__, "mat_pool": pool}))
+    assert normwise(mutated, numpy_kronmult(b)) > 1e-3
+
+
+def test_planted_reversed_factor_order_is_caught():
+    b = W.random_case(26, 3, 4, 3)
+    moff = b.mat_off.reshape(b.nb, b.d)[:, ::-1].reshape(-1).copy()
+    mutated = oracle.kronmult(W.HostBatch(**{**b.__dict__, "mat_off": moff}))
+    assert normwise(mutated, numpy_kronmult(b)) > 1e-3
