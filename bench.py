@@ -145,7 +145,7 @@ def main() -> int:
     ms_per_step = total_ms / args.steps
     k_avg = statistics.mean(kernel_ms)
     log(f"[bench] {args.steps} steps: {ms_per_step * 1e3:.2f} us/step (events), kernel avg {k_avg * 1e3:.2f} us, "
-        f"min {min(kernel_ms) * 1e3:.2f} us, wall {wall * 1e3 / args.steps:.2f} us/step")
+        f"min {min(kernel_ms) * 1e3:.2f} us, wall {wall * 1e6 / args.steps:.2f} us/step")
 
     flops = hb.flops()
     contract = hb.contract_bytes()

@@ -1,0 +1,292 @@
+// kron_reg.hpp -- register-resident sweep for one (d, n, RHO) plan (layout in kron_plan.hpp).
+//
+// One workgroup of P = n^(d-RHO) threads walks a contiguous run of batch entries. Per entry:
+//   1. the input vector, prefetched into registers one entry ahead with 16-byte coalesced loads, is stored
+//      to LDS (swizzled canonical order) and read back in the phase-0 layout (each thread: one row of
+//      R = n^RHO consecutive elements, i.e. the RHO fastest modes);
+//   2. each phase contracts the modes of its register block that are due, fastest first, entirely in
+//      registers: an n-term fused multiply-add chain per output, k ascending from zero, coefficients
+//      uniform across the workgroup (scalar loads) -- bit-identical to the reference's step
+//      (kronmult_gpu/kronmult.cu:54-78, taken with fused multiply-adds);
+//   3. between phases the tensor goes through LDS (one write, one read per element);
+//   4. the final layout is output element j*P + t, so the result is added to a register accumulator; when
+//      the next entry targets another output (or the run ends) the accumulator is flushed with one
+//      hardware atomic add per element (kronmult_gpu/kronmult.cu:124-129 does one per element per entry).
+// Barriers only order LDS traffic (lds_barrier), so prefetched global loads stay in flight across them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kron_plan.hpp"
+
+namespace kronreg
+{
+
+using kronplan::Plan;
+using kronplan::PlanSwizzle;
+
+// Address-space views of caller pointers. Inputs and outputs are device memory (global address space:
+// global_load / global_atomic instead of flat instructions, which would also count against the LDS wait
+// counter). Factor matrices are read-only for the whole launch and uniform across the workgroup, so they
+// are read through the constant address space, which lets the compiler use scalar loads into SGPRs.
+template<typename T>
+using global_ptr = __attribute__((address_space(1))) T *;
+template<typename T>
+using constant_ptr = __attribute__((address_space(4))) T const *;
+
+template<typename T>
+__device__ __forceinline__ global_ptr<T> as_global(T *p)
+{
+    return (global_ptr<T>)p;
+}
+template<typename T>
+__device__ __forceinline__ global_ptr<T const> as_global(T const *p)
+{
+    return (global_ptr<T const>)p;
+}
+template<typename T>
+__device__ __forceinline__ constant_ptr<T> as_constant(T const *p)
+{
+    return (constant_ptr<T>)p;
+}
+
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template<typename T>
+struct Granule
+{
+    static constexpr int G = 16 / sizeof(T);
+    typedef T type __attribute__((ext_vector_type(16 / sizeof(T))));
+};
+
+// contract register mode with stride S in the register index (n-term fused dot per output)
+template<typename T, int N, int R, int S>
+__device__ __forceinline__ void contract_mode(T (&v)[R], T const *A_generic, int stride)
+{
+    constant_ptr<T> A = as_constant(A_generic);
+    T a[N][N];
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+#pragma unroll
+        for (int k = 0; k < N; ++k) a[r][k] = A[r + k * stride];
+#pragma unroll
+    for (int base = 0; base < R; ++base)
+    {
+        if ((base / S) % N != 0) continue;
+        T x[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) x[k] = v[base + k * S];
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+        {
+            T acc = T(0);
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc = fma(x[k], a[r][k], acc);
+            v[base + r * S] = acc;
+        }
+    }
+}
+
+// contract modes M, M-1, ..., LO of phase PH (fastest first)
+template<typename T, typename PL, int PH, int M>
+__device__ __forceinline__ void contract_phase(T (&v)[PL::R], T const *const *__restrict__ A_list, int stride)
+{
+    if constexpr (M >= PL::lo(PH))
+    {
+        contract_mode<T, PL::kN, PL::R, PL::reg_stride(M - PL::lo(PH))>(v, A_list[M], stride);
+        contract_phase<T, PL, PH, M - 1>(v, A_list, stride);
+    }
+}
+
+template<typename T, typename PL>
+struct Lds
+{
+    using SW = PlanSwizzle<PL, T>;
+    static constexpr int G = Granule<T>::G;
+    static constexpr bool kVec = PL::R >= G;
+    using V = typename Granule<T>::type;
+
+    // input store: thread t holds granules c*P + t (element c*P + t when rows are shorter than a granule)
+    static __device__ __forceinline__ void store_input(T *lds, T const (&pre)[PL::R], int t)
+    {
+        if constexpr (kVec)
+        {
+#pragma unroll
+            for (int c = 0; c < PL::R / G; ++c)
+            {
+                V g;
+#pragma unroll
+                for (int i = 0; i < G; ++i) g[i] = pre[c * G + i];
+                *reinterpret_cast<V *>(lds + SW::phys((c * PL::P + t) * G)) = g;
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int c = 0; c < PL::R; ++c) lds[SW::phys(c * PL::P + t)] = pre[c];
+        }
+    }
+
+    template<int PH>
+    static __device__ __forceinline__ void read(T const *lds, T (&v)[PL::R], int t)
+    {
+        if constexpr (PH == 0 && kVec)
+        {
+#pragma unroll
+            for (int q = 0; q < PL::R / G; ++q)
+            {
+                V g = *reinterpret_cast<V const *>(lds + SW::phys(PL::elem(0, t, q * G)));
+#pragma unroll
+                for (int i = 0; i < G; ++i) v[q * G + i] = g[i];
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int j = 0; j < PL::R; ++j) v[j] = lds[SW::phys(PL::elem(PH, t, j))];
+        }
+    }
+
+    template<int PH>
+    static __device__ __forceinline__ void write(T *lds, T const (&v)[PL::R], int t)
+    {
+        if constexpr (PH == 0 && kVec)
+        {
+#pragma unroll
+            for (int q = 0; q < PL::R / G; ++q)
+            {
+                V g;
+#pragma unroll
+                for (int i = 0; i < G; ++i) g[i] = v[q * G + i];
+                *reinterpret_cast<V *>(lds + SW::phys(PL::elem(0, t, q * G))) = g;
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int j = 0; j < PL::R; ++j) lds[SW::phys(PL::elem(PH, t, j))] = v[j];
+        }
+    }
+};
+
+// prefetch the input of one entry: thread t loads granule c*P + t for each c (16-byte loads when the
+// pointer is 16-byte aligned, element loads otherwise)
+template<typename T, typename PL>
+__device__ __forceinline__ void load_input(T const *src_generic, T (&pre)[PL::R], int t)
+{
+    constexpr int G = Granule<T>::G;
+    global_ptr<T const> src = as_global(src_generic);
+    if constexpr (PL::R >= G)
+    {
+        using V = typename Granule<T>::type;
+        if ((reinterpret_cast<uintptr_t>(src_generic) & 15) == 0)
+        {
+#pragma unroll
+            for (int c = 0; c < PL::R / G; ++c)
+            {
+                V g = *(global_ptr<V const>)(src + (c * PL::P + t) * G);
+#pragma unroll
+                for (int i = 0; i < G; ++i) pre[c * G + i] = g[i];
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int c = 0; c < PL::R / G; ++c)
+#pragma unroll
+                for (int i = 0; i < G; ++i) pre[c * G + i] = src[(c * PL::P + t) * G + i];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int c = 0; c < PL::R; ++c) pre[c] = src[c * PL::P + t];
+    }
+}
+
+template<typename T, typename PL, int PH>
+__device__ __forceinline__ void run_phases(T *lds, T (&v)[PL::R], T const *const *__restrict__ A_list, int stride,
+                                           int t)
+{
+    contract_phase<T, PL, PH, PL::c_hi(PH) - 1>(v, A_list, stride);
+    if constexpr (PH + 1 < PL::NPHASE)
+    {
+        lds_barrier();  // every thread has read phase PH
+        Lds<T, PL>::template write<PH>(lds, v, t);
+        lds_barrier();
+        Lds<T, PL>::template read<PH + 1>(lds, v, t);
+        run_phases<T, PL, PH + 1>(lds, v, A_list, stride, t);
+    }
+}
+
+template<typename T, typename PL>
+__device__ __forceinline__ void flush(T *dst_generic, T (&acc)[PL::R], int t)
+{
+    global_ptr<T> dst = as_global(dst_generic);
+#pragma unroll
+    for (int j = 0; j < PL::R; ++j)
+    {
+        atomicAdd((T *)(dst + j * PL::P + t), acc[j]);
+        acc[j] = T(0);
+    }
+}
+
+// Workgroup indices are dealt to the 8 XCDs round-robin (XCD = blockIdx % 8). Workgroup b takes the entry
+// run of logical index (b % 8) * (grid / 8) + b / 8, so each XCD walks one contiguous stretch of the batch:
+// entries that share inputs and outputs (consecutive in the callers' orderings) stay on one XCD's L2.
+// Logical run w covers entries [w*epw, min(nb, (w+1)*epw)); the grid is a multiple of 8.
+__device__ __forceinline__ int xcd_run_index()
+{
+    int const per_xcd = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+}
+
+template<typename T, typename PL>
+__global__ void __launch_bounds__(PL::P)
+    kron_reg_kernel(int nb, int epw, T const *const *__restrict__ mats, int stride, T const *const *__restrict__ in,
+                    T *const *__restrict__ out)
+{
+    static_assert(PL::NPHASE >= 1, "");
+    __shared__ __attribute__((aligned(16))) T lds[PL::SIZE];
+    int const t = threadIdx.x;
+    int const e0 = xcd_run_index() * epw;
+    int const e1 = (e0 + epw < nb) ? e0 + epw : nb;
+    if (e0 >= e1) return;
+
+    T acc[PL::R];
+#pragma unroll
+    for (int j = 0; j < PL::R; ++j) acc[j] = T(0);
+    T pre[PL::R];
+    T v[PL::R];
+    load_input<T, PL>(in[e0], pre, t);
+    T *cur = out[e0];
+
+    for (int e = e0; e < e1; ++e)
+    {
+        lds_barrier();  // LDS free: the previous entry's last phase has been read by everyone
+        Lds<T, PL>::store_input(lds, pre, t);
+        lds_barrier();
+        if (e + 1 < e1) load_input<T, PL>(in[e + 1], pre, t);
+        Lds<T, PL>::template read<0>(lds, v, t);
+        run_phases<T, PL, 0>(lds, v, mats + static_cast<size_t>(e) * PL::kD, stride, t);
+
+        T *dst = out[e];
+        if (dst != cur)
+        {
+            flush<T, PL>(cur, acc, t);
+            cur = dst;
+        }
+#pragma unroll
+        for (int j = 0; j < PL::R; ++j) acc[j] += v[j];
+    }
+    flush<T, PL>(cur, acc, t);
+}
+
+} // namespace kronreg

@@ -10,15 +10,20 @@
 // are bit-identical to the reference sweep evaluated with fused multiply-adds.
 //
 // Kernel families:
+//   * register-resident sweep (kron_reg.hpp) for the (d, n) plans listed in reg_plan(): the tensor lives
+//     in registers, n^RHO values per thread, with LDS only between phases; a workgroup walks a run of
+//     entries and accumulates same-output contributions in registers before one atomic flush;
 //   * fiber sweep (any d, n <= 64): the n^d vector lives in LDS (<= 64 KiB) or, beyond that, in the
 //     caller's per-entry workspace; one workgroup per entry; shared outputs are accumulated with
 //     hardware FP64/FP32 atomics (global_atomic_add_f64/f32).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <climits>
 #include <cstdint>
 #include <cstdio>
 
+#include "kron_reg.hpp"
 #include "kronmult_hip.h"
 
 namespace
@@ -143,14 +148,24 @@ hipError_t validate(int d, int n, int stride, int nb, Shape &s)
 
 enum class Path
 {
+    Reg,
     FiberLds,
     FiberGlobal,
     Unsupported
 };
 
+// Register-resident plans (d, n) -> RHO. Threads per workgroup P = n^(d-RHO) must be a multiple of 64
+// and at most 1024; n^d elements per entry stay in LDS between phases.
+constexpr int reg_rho(int d, int n)
+{
+    return (d == 6 && n == 4) ? 2 : (d == 5 && n == 4) ? 2 : (d == 4 && n == 4) ? 1
+         : (d == 4 && n == 8) ? 1 : (d == 3 && n == 8) ? 1 : 0;
+}
+
 template<typename T>
 Path choose_path(Shape const &s)
 {
+    if (reg_rho(s.d, s.n) > 0) return Path::Reg;
     if (s.n > kMaxFiber) return Path::Unsupported;
     long const coef_bytes = (s.n <= 8) ? 0 : static_cast<long>(s.n) * s.n * sizeof(T);
     if (s.size * static_cast<long>(sizeof(T)) + coef_bytes <= kLdsBudget) return Path::FiberLds;
@@ -190,6 +205,66 @@ hipError_t dispatch_fiber_n(Shape const &s, T const *const *mats, T const *const
     }
 }
 
+// Number of compute units of the current device (cached per device; the value never changes).
+int device_cu_count()
+{
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (v == 0)
+    {
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cache[dev].store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
+
+// Resident workgroups per CU of a kernel (cached per kernel; the occupancy does not depend on the device
+// beyond the architecture this library is built for).
+template<typename Kernel>
+int resident_per_cu(Kernel kernel, int block)
+{
+    static std::atomic<int> cache{0};
+    int v = cache.load(std::memory_order_relaxed);
+    if (v == 0)
+    {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, block, 0) != hipSuccess || v <= 0) v = 1;
+        cache.store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
+
+// The register-resident sweep is launched as exactly one resident round: grid = CUs x resident
+// workgroups per CU (a multiple of 8, so every XCD gets the same number of runs), and each workgroup walks
+// a run of ceil(nb / grid) consecutive entries.
+template<typename T, int D, int N>
+hipError_t launch_reg(Shape const &s, T const *const *mats, T const *const *in, T *const *out, hipStream_t stream)
+{
+    using PL = kronplan::Plan<D, N, reg_rho(D, N)>;
+    static_assert(PL::P % 64 == 0 && PL::P <= 1024, "plan must fill whole wavefronts");
+    auto kernel = kronreg::kron_reg_kernel<T, PL>;
+    long const slots = static_cast<long>(device_cu_count()) * resident_per_cu(kernel, PL::P);
+    int epw = static_cast<int>((s.nb + slots - 1) / slots);
+    if (epw < 1) epw = 1;
+    long runs = (s.nb + epw - 1) / epw;
+    runs = (runs + 7) / 8 * 8;  // whole rounds over the 8 XCDs; surplus runs exit at once
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(runs)), dim3(PL::P), 0, stream, s.nb, epw, mats, s.stride,
+                       in, out);
+    return hipGetLastError();
+}
+
+template<typename T>
+hipError_t dispatch_reg(Shape const &s, T const *const *mats, T const *const *in, T *const *out, hipStream_t stream)
+{
+    if (s.d == 6 && s.n == 4) return launch_reg<T, 6, 4>(s, mats, in, out, stream);
+    if (s.d == 5 && s.n == 4) return launch_reg<T, 5, 4>(s, mats, in, out, stream);
+    if (s.d == 4 && s.n == 4) return launch_reg<T, 4, 4>(s, mats, in, out, stream);
+    if (s.d == 4 && s.n == 8) return launch_reg<T, 4, 8>(s, mats, in, out, stream);
+    if (s.d == 3 && s.n == 8) return launch_reg<T, 3, 8>(s, mats, in, out, stream);
+    return hipErrorInvalidValue;
+}
+
 template<typename T>
 hipError_t kronmult_launch(int d, int n, T const *const *mats, int stride, T *const *in, T *const *out,
                            T *const *ws, int nb, hipStream_t stream)
@@ -202,6 +277,7 @@ hipError_t kronmult_launch(int d, int n, T const *const *mats, int stride, T *co
     T const *const *cin = reinterpret_cast<T const *const *>(in);
     switch (choose_path<T>(s))
     {
+    case Path::Reg: return dispatch_reg<T>(s, mats, cin, out, stream);
     case Path::FiberLds: return dispatch_fiber_n<T, true>(s, mats, cin, out, ws, stream);
     case Path::FiberGlobal:
         if (ws == nullptr) return hipErrorInvalidValue;
@@ -229,6 +305,7 @@ char const *kernel_name(int d, int n, int nb)
     bool const f64 = sizeof(T) == 8;
     switch (choose_path<T>(s))
     {
+    case Path::Reg: return f64 ? "reg_sweep_f64" : "reg_sweep_f32";
     case Path::FiberLds: return f64 ? "fiber_lds_f64" : "fiber_lds_f32";
     case Path::FiberGlobal: return f64 ? "fiber_global_f64" : "fiber_global_f32";
     default: return "unsupported";

@@ -12,7 +12,6 @@ import numpy as np
 from . import api
 from .workloads import HostBatch
 
-_LDS_BYTES = 64 * 1024  # vectors above this size run from the per-entry workspace
 
 
 def _torch():
@@ -41,7 +40,8 @@ class DeviceBatch:
         self.mat_ptrs = self._ptrs(self.mat, hb.mat_off)
         self.in_ptrs = self._ptrs(self.inp, hb.in_off)
         self.out_ptrs = self._ptrs(self.out, hb.out_off)
-        need_ws = workspace == "per_entry" or (workspace == "auto" and self.size * self.itemsize > _LDS_BYTES)
+        uses_ws = api.kernel_name(self.d, self.n, self.itemsize, self.nb).startswith("fiber_global")
+        need_ws = workspace == "per_entry" or (workspace == "auto" and uses_ws)
         if need_ws:
             self.ws = torch.zeros(self.nb * self.size, dtype=self.out.dtype, device=self.device)
             self.ws_ptrs = self._ptrs(self.ws, np.arange(self.nb, dtype=np.int64) * self.size)

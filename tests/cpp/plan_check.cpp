@@ -1,0 +1,226 @@
+// plan_check.cpp -- CPU checks of the register-resident sweep layout (kronmult993_amd/csrc/kron_plan.hpp).
+//
+// For every plan the GPU library instantiates, and both element sizes:
+//   1. each phase layout elem(p, t, j) is a bijection of the n^d elements;
+//   2. the LDS swizzle is a bijection;
+//   3. every LDS access pattern of the kernel (input store, phase reads/writes) is free of bank
+//      conflicts in the model of gfx950's LDS: 64 banks x 4 bytes, 256 bytes per cycle, lanes served in
+//      groups of 256 bytes (the 32-bank count is printed for reference);
+//   4. the phase schedule, emulated thread by thread with fused multiply-adds, reproduces the oracle's
+//      sweep (oracle/liboracle.so) bit for bit.
+// Exit status 0 when everything holds; the pytest wrapper runs it on the CPU.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <set>
+#include <vector>
+
+#include "kron_plan.hpp"
+
+extern "C" int oracle_kronmult_batched_f64(int d, int n, double const *const *mats, int stride,
+                                           double const *const *in, double *const *out, int nb, int fused);
+extern "C" int oracle_kronmult_batched_f32(int d, int n, float const *const *mats, int stride,
+                                           float const *const *in, float *const *out, int nb, int fused);
+
+using namespace kronplan;
+
+static int g_fail = 0;
+#define CHECK(cond, ...)                                                                                      \
+    do                                                                                                         \
+    {                                                                                                          \
+        if (!(cond))                                                                                           \
+        {                                                                                                      \
+            std::printf("FAIL: " __VA_ARGS__);                                                                 \
+            std::printf("\n");                                                                                 \
+            ++g_fail;                                                                                          \
+        }                                                                                                      \
+    } while (0)
+
+// cycles one wavefront instruction takes in the bank model: lanes are served in groups whose byte count
+// equals the LDS width; within a group, a bank hit by k distinct dwords costs k cycles.
+static int access_cycles(std::vector<long> const &byte_addr, int bytes_per_lane, int nbanks)
+{
+    int const width = nbanks * 4;
+    int const lanes_per_group = width / bytes_per_lane;
+    int cycles = 0;
+    for (size_t g0 = 0; g0 < byte_addr.size(); g0 += lanes_per_group)
+    {
+        std::vector<std::set<long>> bank(nbanks);
+        for (size_t l = g0; l < g0 + lanes_per_group && l < byte_addr.size(); ++l)
+            for (int w = 0; w < bytes_per_lane / 4; ++w)
+            {
+                long const dword = byte_addr[l] / 4 + w;
+                bank[dword % nbanks].insert(dword);
+            }
+        size_t worst = 1;
+        for (auto const &b : bank) worst = b.size() > worst ? b.size() : worst;
+        cycles += static_cast<int>(worst);
+    }
+    return cycles;
+}
+
+struct ConflictReport
+{
+    int cycles = 0, ideal = 0;
+};
+
+template<typename PL, typename T>
+void check_plan(char const *name, bool conflict_free)
+{
+    constexpr int G = 16 / sizeof(T);
+    using SW = PlanSwizzle<PL, T>;
+    int const S = PL::SIZE, R = PL::R, P = PL::P;
+    std::printf("plan %-10s %s: SIZE=%d R=%d P=%d phases=%d\n", name, sizeof(T) == 8 ? "f64" : "f32", S, R, P,
+                PL::NPHASE);
+
+    // 1. bijections
+    for (int p = 0; p < PL::NPHASE; ++p)
+    {
+        std::vector<int> seen(S, 0);
+        for (int t = 0; t < P; ++t)
+            for (int j = 0; j < R; ++j) seen[PL::elem(p, t, j)]++;
+        bool ok = true;
+        for (int e = 0; e < S; ++e) ok = ok && seen[e] == 1;
+        CHECK(ok, "%s phase %d layout is not a bijection", name, p);
+    }
+    {
+        std::vector<int> seen(S, 0);
+        for (int e = 0; e < S; ++e)
+        {
+            int const q = SW::phys(e);
+            CHECK(q >= 0 && q < S, "%s swizzle out of range", name);
+            if (q >= 0 && q < S) seen[q]++;
+        }
+        bool ok = true;
+        for (int e = 0; e < S; ++e) ok = ok && seen[e] == 1;
+        CHECK(ok, "%s swizzle is not a bijection", name);
+        // granules stay whole
+        for (int e = 0; e < S; e += G)
+            for (int k = 1; k < G && R >= G; ++k)
+                CHECK(SW::phys(e + k) == SW::phys(e) + k, "%s swizzle splits a granule at %d", name, e);
+    }
+
+    // 3. bank conflicts, per wavefront (64 lanes) of the workgroup
+    for (int nbanks : {32, 64})
+    {
+        ConflictReport rep;
+        auto run = [&](std::vector<long> const &addr, int bytes) {
+            rep.cycles += access_cycles(addr, bytes, nbanks);
+            rep.ideal += static_cast<int>((addr.size() * bytes + nbanks * 4 - 1) / (nbanks * 4));
+        };
+        int const waves = (P + 63) / 64;
+        for (int w = 0; w < waves; ++w)
+        {
+            int const lanes = P - 64 * w < 64 ? P - 64 * w : 64;
+            if (R >= G)
+            {
+                // input store: thread t stores granule c*P + t (coalesced global load order)
+                for (int c = 0; c < R / G; ++c)
+                {
+                    std::vector<long> a;
+                    for (int l = 0; l < lanes; ++l)
+                        a.push_back(static_cast<long>(SW::phys((c * P + 64 * w + l) * G)) * sizeof(T));
+                    run(a, 16);
+                }
+                // phase 0: thread t owns row t, reads/writes granule q
+                for (int q = 0; q < R / G; ++q)
+                {
+                    std::vector<long> a;
+                    for (int l = 0; l < lanes; ++l)
+                        a.push_back(static_cast<long>(SW::phys(PL::elem(0, 64 * w + l, q * G))) * sizeof(T));
+                    run(a, 16);
+                    run(a, 16);
+                }
+            }
+            // later phases: element-wise
+            for (int p = 1; p < PL::NPHASE; ++p)
+                for (int j = 0; j < R; ++j)
+                {
+                    std::vector<long> a;
+                    for (int l = 0; l < lanes; ++l)
+                        a.push_back(static_cast<long>(SW::phys(PL::elem(p, 64 * w + l, j))) * sizeof(T));
+                    run(a, sizeof(T));
+                    if (p + 1 < PL::NPHASE) run(a, sizeof(T));
+                }
+        }
+        std::printf("    LDS %2d banks: %d cycles for %d ideal (x%.2f)\n", nbanks, rep.cycles, rep.ideal,
+                    double(rep.cycles) / rep.ideal);
+        if (nbanks == 64 && conflict_free)
+            CHECK(rep.cycles == rep.ideal, "%s has LDS bank conflicts (64 banks)", name);
+    }
+
+    // 4. emulate the schedule and compare with the oracle
+    int const d = PL::kD, n = PL::kN;
+    std::mt19937_64 rng(1234 + S + R);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    int const stride = n + 3;
+    std::vector<T> A(static_cast<size_t>(d) * n * stride);
+    for (auto &a : A) a = static_cast<T>(U(rng));
+    std::vector<T> x(S);
+    for (auto &v : x) v = static_cast<T>(U(rng));
+    std::vector<T const *> mats(d);
+    for (int t = 0; t < d; ++t) mats[t] = A.data() + static_cast<size_t>(t) * n * stride;
+
+    std::vector<T> tensor(x), regs(static_cast<size_t>(P) * R);
+    for (int p = 0; p < PL::NPHASE; ++p)
+    {
+        for (int t = 0; t < P; ++t)
+            for (int j = 0; j < R; ++j) regs[static_cast<size_t>(t) * R + j] = tensor[PL::elem(p, t, j)];
+        for (int m = PL::c_hi(p) - 1; m >= PL::lo(p); --m)
+        {
+            int const s = PL::reg_stride(m - PL::lo(p));
+            T const *Am = mats[m];
+            for (int t = 0; t < P; ++t)
+            {
+                T *v = regs.data() + static_cast<size_t>(t) * R;
+                for (int base = 0; base < R; ++base)
+                {
+                    if ((base / s) % n != 0) continue;
+                    T f[64];
+                    for (int k = 0; k < n; ++k) f[k] = v[base + k * s];
+                    for (int r = 0; r < n; ++r)
+                    {
+                        T acc = 0;
+                        for (int k = 0; k < n; ++k) acc = std::fma(f[k], Am[r + k * stride], acc);
+                        v[base + r * s] = acc;
+                    }
+                }
+            }
+        }
+        for (int t = 0; t < P; ++t)
+            for (int j = 0; j < R; ++j) tensor[PL::elem(p, t, j)] = regs[static_cast<size_t>(t) * R + j];
+    }
+    std::vector<T> ref(S, T(0));
+    T const *inp = x.data();
+    T *outp = ref.data();
+    if constexpr (sizeof(T) == 8)
+        oracle_kronmult_batched_f64(d, n, mats.data(), stride, &inp, &outp, 1, 1);
+    else
+        oracle_kronmult_batched_f32(d, n, mats.data(), stride, &inp, &outp, 1, 1);
+    CHECK(std::memcmp(ref.data(), tensor.data(), sizeof(T) * S) == 0, "%s emulation differs from the oracle",
+          name);
+}
+
+// conflict_free: the bank-conflict count must be ideal in the 64-bank model
+template<int D, int N, int RHO>
+void check(char const *name, bool f64_conflict_free, bool f32_conflict_free)
+{
+    check_plan<Plan<D, N, RHO>, double>(name, f64_conflict_free);
+    check_plan<Plan<D, N, RHO>, float>(name, f32_conflict_free);
+}
+
+int main()
+{
+    check<6, 4, 2>("d6n4r2", true, true);
+    check<6, 4, 3>("d6n4r3", true, true);
+    check<4, 8, 1>("d4n8r1", true, true);
+    check<4, 8, 2>("d4n8r2", true, true);
+    check<3, 8, 1>("d3n8r1", true, true);
+    check<3, 8, 2>("d3n8r2", true, true);
+    check<5, 4, 2>("d5n4r2", true, true);
+    check<4, 4, 1>("d4n4r1", true, true);
+    std::printf(g_fail ? "plan_check: %d failures\n" : "plan_check: all checks passed\n", g_fail);
+    return g_fail ? 1 : 0;
+}

@@ -121,6 +121,24 @@ def test_random_shapes(gpu, d, n, nb, seed):
         _check_bitexact(u, _run(u, gpu))
 
 
+@pytest.mark.parametrize("d,n", [(6, 4), (5, 4), (4, 4), (4, 8), (3, 8)])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_register_plans_irregular(gpu, d, n, dtype):
+    """The register-resident plans with padded strides, scrambled aliased outputs, shared factors and inputs
+    whose addresses are only element-aligned (odd entries shifted by one element)."""
+    b = W.random_case(60 + d * n, d, n, 97, stride=n + 3, shared_factors=13).astype(dtype)
+    assert api.kernel_name(d, n, b.dtype.itemsize, b.nb).startswith("reg_sweep")
+    size = b.size
+    pool = np.zeros(b.nb * (size + 1), dtype=b.dtype)
+    in_off = np.arange(b.nb, dtype=np.int64) * (size + 1) + (np.arange(b.nb) % 2)
+    for k in range(b.nb):
+        pool[in_off[k]: in_off[k] + size] = b.in_pool[b.in_off[k]: b.in_off[k] + size]
+    b = W.HostBatch(**{**b.__dict__, "in_pool": pool, "in_off": in_off})
+    _check_aliased(b, _run(b, gpu))
+    u = _unaliased(b)
+    _check_bitexact(u, _run(u, gpu))
+
+
 @pytest.mark.parametrize("d,n,nb", [(3, 21, 6), (6, 7, 3), (4, 12, 4)])
 def test_large_vectors_use_workspace(gpu, d, n, nb):
     """n^d * 8 bytes above the on-chip size: the sweep runs from the caller's workspace."""
