@@ -50,6 +50,53 @@ int kronmult_batched_f32_async(int matrix_count, int matrix_size, float const *c
                                float **workspace_batched, int nb_batch, void *stream);
 
 /*
+ * Launch-shape control for the stream-ordered entry points (no counterpart in the reference, which has a
+ * single launch shape, kronmult_gpu/kronmult.cu:184-193). Zero fields mean "library default".
+ *   family:                 KRONMULT_FAMILY_AUTO, _REGISTER (register-resident sweep, only for the
+ *                           (d, n) plans it is built for) or _FIBER (fiber sweep, any n <= 64);
+ *   workgroups_per_cu:      resident workgroups to launch per compute unit (register family);
+ *   entries_per_workgroup:  consecutive entries one workgroup walks (register family).
+ * An impossible request (a family that does not cover (d, n)) returns hipErrorInvalidValue.
+ */
+#define KRONMULT_FAMILY_AUTO 0
+#define KRONMULT_FAMILY_REGISTER 1
+#define KRONMULT_FAMILY_FIBER 2
+
+typedef struct kronmult_tuning
+{
+    int family;
+    int workgroups_per_cu;
+    int entries_per_workgroup;
+} kronmult_tuning_t;
+
+typedef struct kronmult_launch_info
+{
+    int family;                /* KRONMULT_FAMILY_REGISTER or _FIBER */
+    int grid;                  /* workgroups launched */
+    int block;                 /* threads per workgroup */
+    int entries_per_workgroup; /* entries walked by one workgroup */
+    int lds_bytes;             /* LDS per workgroup */
+    int resident_per_cu;       /* workgroups per CU the occupancy calculator allows */
+    int uses_workspace;        /* 1 if workspace_batched is read (vector too large for LDS) */
+} kronmult_launch_info_t;
+
+int kronmult_batched_f64_tuned(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                               int matrix_stride, double **input_batched, double **output_batched,
+                               double **workspace_batched, int nb_batch, kronmult_tuning_t const *tuning,
+                               void *stream);
+int kronmult_batched_f32_tuned(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
+                               int matrix_stride, float **input_batched, float **output_batched,
+                               float **workspace_batched, int nb_batch, kronmult_tuning_t const *tuning,
+                               void *stream);
+
+/*
+ * The launch the entry points above would make for (d, n, element size in bytes, nb_batch, tuning) on the
+ * current device. Needs the HIP runtime (occupancy query); returns a hipError_t code.
+ */
+int kronmult_hip_launch_info(int matrix_count, int matrix_size, int element_size, int nb_batch,
+                             kronmult_tuning_t const *tuning, kronmult_launch_info_t *info);
+
+/*
  * Library version string and the name of the kernel the dispatcher would pick for (d, n, element size in
  * bytes, nb_batch); both point to static storage. No reference counterpart (diagnostics only).
  */

@@ -19,12 +19,30 @@ _lock = threading.Lock()
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 
+
+class Tuning(ctypes.Structure):
+    """kronmult_tuning_t: family (0 auto, 1 register, 2 fiber), workgroups per CU, entries per workgroup;
+    zero fields keep the library default."""
+    _fields_ = [("family", _I), ("workgroups_per_cu", _I), ("entries_per_workgroup", _I)]
+
+
+class LaunchInfo(ctypes.Structure):
+    """kronmult_launch_info_t."""
+    _fields_ = [("family", _I), ("grid", _I), ("block", _I), ("entries_per_workgroup", _I), ("lds_bytes", _I),
+                ("resident_per_cu", _I), ("uses_workspace", _I)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
 # name -> (restype, argtypes); the order of arguments is the one of include/kronmult_hip.h
 SIGNATURES = {
     "kronmult_batched_f64": (_I, [_I, _I, _P, _I, _P, _P, _P, _I]),
     "kronmult_batched_f32": (_I, [_I, _I, _P, _I, _P, _P, _P, _I]),
     "kronmult_batched_f64_async": (_I, [_I, _I, _P, _I, _P, _P, _P, _I, _P]),
     "kronmult_batched_f32_async": (_I, [_I, _I, _P, _I, _P, _P, _P, _I, _P]),
+    "kronmult_batched_f64_tuned": (_I, [_I, _I, _P, _I, _P, _P, _P, _I, ctypes.POINTER(Tuning), _P]),
+    "kronmult_batched_f32_tuned": (_I, [_I, _I, _P, _I, _P, _P, _P, _I, ctypes.POINTER(Tuning), _P]),
+    "kronmult_hip_launch_info": (_I, [_I, _I, _I, _I, ctypes.POINTER(Tuning), ctypes.POINTER(LaunchInfo)]),
     "kronmult_hip_version": (ctypes.c_char_p, []),
     "kronmult_hip_kernel_name": (ctypes.c_char_p, [_I, _I, _I, _I]),
 }

@@ -14,7 +14,9 @@ from __future__ import annotations
 
 from typing import Any, Optional
 
-from ._lib import lib
+import ctypes
+
+from ._lib import LaunchInfo, Tuning, lib
 
 HIP_SUCCESS = 0
 HIP_ERROR_INVALID_VALUE = 1
@@ -77,6 +79,43 @@ def kronmult_batched_async(matrix_count: int, matrix_size: int, matrix_list_batc
            _addr(output_batched), _addr(workspace_batched), nb_batch, _stream_handle(stream))
     if rc != HIP_SUCCESS:
         raise KronmultError(rc, "kronmult_batched_async")
+
+
+FAMILY_AUTO, FAMILY_REGISTER, FAMILY_FIBER = 0, 1, 2
+
+
+def _tuning(tuning) -> Optional[Tuning]:
+    if tuning is None:
+        return None
+    if isinstance(tuning, Tuning):
+        return tuning
+    return Tuning(int(tuning.get("family", 0)), int(tuning.get("workgroups_per_cu", 0)),
+                  int(tuning.get("entries_per_workgroup", 0)))
+
+
+def kronmult_batched_tuned(matrix_count: int, matrix_size: int, matrix_list_batched, matrix_stride: int,
+                           input_batched, output_batched, workspace_batched, nb_batch: int, *, dtype,
+                           tuning=None, stream=None) -> None:
+    """Stream-ordered call with an explicit launch shape (dict with family / workgroups_per_cu /
+    entries_per_workgroup, or a Tuning)."""
+    f = getattr(lib(), "kronmult_batched_" + _suffix(dtype) + "_tuned")
+    tu = _tuning(tuning)
+    rc = f(matrix_count, matrix_size, _addr(matrix_list_batched), matrix_stride, _addr(input_batched),
+           _addr(output_batched), _addr(workspace_batched), nb_batch, ctypes.byref(tu) if tu else None,
+           _stream_handle(stream))
+    if rc != HIP_SUCCESS:
+        raise KronmultError(rc, "kronmult_batched_tuned")
+
+
+def launch_info(matrix_count: int, matrix_size: int, element_size: int, nb_batch: int, tuning=None) -> dict:
+    """The launch the library would make (needs the HIP runtime)."""
+    info = LaunchInfo()
+    tu = _tuning(tuning)
+    rc = lib().kronmult_hip_launch_info(matrix_count, matrix_size, element_size, nb_batch,
+                                        ctypes.byref(tu) if tu else None, ctypes.byref(info))
+    if rc != HIP_SUCCESS:
+        raise KronmultError(rc, "kronmult_hip_launch_info")
+    return info.as_dict()
 
 
 def kernel_name(matrix_count: int, matrix_size: int, element_size: int, nb_batch: int) -> str:

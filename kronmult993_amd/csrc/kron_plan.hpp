@@ -56,43 +56,45 @@ struct Plan
 };
 
 // LDS placement of canonical element e. The tensor is stored in 16-byte granules (G = 16 / sizeof(T)
-// elements); granule index g goes to g ^ x(g), where bit i (i < 4) of x is bit src_i of g. gfx950's LDS
-// serves 256 bytes per cycle from 64 four-byte banks, i.e. 16 granules, so the four low granule bits
-// select the bank group. The sources per plan come from an exhaustive search over single-bit XOR maps
-// under that bank model (every plan below is conflict-free for all of its LDS accesses; the CPU check
-// tests/cpp/plan_check.cpp verifies it). Code layout: 5 bits per target bit, value = source bit + 1,
-// 0 = leave the bit alone.
-constexpr unsigned swizzle_code(int d, int n, int rho, int elem_bytes)
+// elements); granule index g goes to g ^ x(g), where bit i (i < 4) of x is the parity of g & mask_i: a
+// linear (GF(2)) map of the granule bits into the four low ones, which pick the bank group.
+//
+// gfx950's LDS was measured (SQ_LDS_BANK_CONFLICT on single-pattern probe kernels) to serve 8- and
+// 16-byte reads as 64 banks x 4 bytes, 256 bytes per cycle (ds_read_b64: 32 lanes per cycle,
+// ds_read_b128: 16 lanes), and 8- and 16-byte writes as 32 banks, 128 bytes per cycle (ds_write_b64: 16
+// lanes, ds_write_b128: 8 lanes). The masks below were found by search so that every LDS read and write
+// of each plan is conflict-free in that model; tests/cpp/plan_check.cpp verifies it for each plan.
+// Layout: mask_i in bits [16 i, 16 i + 16).
+constexpr unsigned long long swizzle_masks(int d, int n, int rho, int elem_bytes)
 {
-    return (d == 6 && n == 4 && rho == 2) ? 0x41cc5u
-         : (d == 5 && n == 4 && rho == 2) ? 0x41cc5u
-         : (d == 4 && n == 4 && rho == 1) ? (elem_bytes == 8 ? 0x318a5u : 0x314c5u)
-         : (d == 4 && n == 8 && rho == 1) ? (elem_bytes == 8 ? 0x398c5u : 0x398a5u)
-         : (d == 3 && n == 8 && rho == 1) ? (elem_bytes == 8 ? 0x398c5u : 0x398a5u)
-         : (d == 4 && n == 8 && rho == 2) ? (elem_bytes == 8 ? 0x4a0e6u : 0x41cc5u)
-         : (d == 6 && n == 4 && rho == 3) ? (elem_bytes == 8 ? 0x4a0e6u : 0x41cc5u)
-         : (d == 3 && n == 8 && rho == 2) ? (elem_bytes == 8 ? 0x020e6u : 0x01cc5u)
-         : 0u;
+    return (d == 6 && n == 4 && rho == 2) ? (elem_bytes == 8 ? 0x00c0002000100008ull : 0x0020004000100008ull)
+         : (d == 5 && n == 4 && rho == 2) ? (elem_bytes == 8 ? 0x00c0002000100008ull : 0x0020004000100008ull)
+         : (d == 4 && n == 4 && rho == 1) ? (elem_bytes == 8 ? 0x0020001000080006ull : 0x0000001000080004ull)
+         : (d == 4 && n == 8 && rho == 1) ? (elem_bytes == 8 ? 0x0044002000100008ull : 0x0002002000100008ull)
+         : (d == 3 && n == 8 && rho == 1) ? (elem_bytes == 8 ? 0x0044002000100008ull : 0x0002002000100008ull)
+         : (d == 4 && n == 8 && rho == 2) ? (elem_bytes == 8 ? 0x0100008000400020ull : 0x0080004000200010ull)
+         : (d == 6 && n == 4 && rho == 3) ? (elem_bytes == 8 ? 0x0100008000400020ull : 0x0080004000200010ull)
+         : 0ull;
 }
 
-template<unsigned CODE, int G>
+template<unsigned long long MASKS, int G>
 struct Swizzle
 {
     static constexpr int phys(int e)
     {
-        int const g = e / G;
-        int x = 0;
+        unsigned const g = static_cast<unsigned>(e / G);
+        unsigned x = 0;
         for (int i = 0; i < 4; ++i)
         {
-            int const src = static_cast<int>((CODE >> (5 * i)) & 31u);
-            if (src) x |= ((g >> (src - 1)) & 1) << i;
+            unsigned const m = static_cast<unsigned>((MASKS >> (16 * i)) & 0xffffull);
+            x |= static_cast<unsigned>(__builtin_parity(g & m)) << i;
         }
-        return (g ^ x) * G + e % G;
+        return static_cast<int>(g ^ x) * G + e % G;
     }
 };
 
 template<typename PL, typename T>
-using PlanSwizzle = Swizzle<swizzle_code(PL::kD, PL::kN, PL::kRho, static_cast<int>(sizeof(T))),
+using PlanSwizzle = Swizzle<swizzle_masks(PL::kD, PL::kN, PL::kRho, static_cast<int>(sizeof(T))),
                             static_cast<int>(16 / sizeof(T))>;
 
 } // namespace kronplan

@@ -94,14 +94,86 @@ __device__ __forceinline__ void contract_mode(T (&v)[R], T const *A_generic, int
     }
 }
 
+// Same contraction with the coefficients staged in LDS as rows a[r][0..N-1] (all lanes read the same
+// address: a broadcast). Output row r of every fiber is formed before the next row is read, from copies
+// of the fiber inputs, so only one coefficient row is live at a time.
+template<typename T, int N, int R, int S>
+__device__ __forceinline__ void contract_mode_lds(T (&v)[R], T const *coef)
+{
+    T x[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) x[j] = v[j];
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+    {
+        T a[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) a[k] = coef[r * N + k];
+#pragma unroll
+        for (int base = 0; base < R; ++base)
+        {
+            if ((base / S) % N != 0) continue;
+            T acc = T(0);
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc = fma(x[base + k * S], a[k], acc);
+            v[base + r * S] = acc;
+        }
+    }
+}
+
+// Where the factors are read from during the phases: straight from the caller's matrices with scalar
+// loads (small n: a factor fits in SGPRs), or from a copy staged in LDS one entry ahead (n >= 8: a
+// factor is 64+ values, and per-entry factors would otherwise stall every phase on a memory round trip).
+template<typename PL>
+constexpr bool coef_in_lds()
+{
+    return PL::kN >= 8;
+}
+
+template<typename T, typename PL>
+struct Coef
+{
+    static constexpr int kCount = PL::kD * PL::kN * PL::kN;                  // staged values per entry
+    static constexpr int kPerThread = (kCount + PL::P - 1) / PL::P;
+    // staging layout: factor m, row r, column k at m*N*N + r*N + k
+    static __device__ __forceinline__ void load(T const *const *A_list, int stride, T (&pre)[kPerThread], int t)
+    {
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i)
+        {
+            int const idx = t + i * PL::P;
+            if (kCount % PL::P == 0 || idx < kCount)
+            {
+                int const m = idx / (PL::kN * PL::kN);
+                int const r = (idx / PL::kN) % PL::kN;
+                int const k = idx % PL::kN;
+                pre[i] = as_global(A_list[m])[r + k * stride];
+            }
+        }
+    }
+    static __device__ __forceinline__ void store(T *coef, T const (&pre)[kPerThread], int t)
+    {
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i)
+        {
+            int const idx = t + i * PL::P;
+            if (kCount % PL::P == 0 || idx < kCount) coef[idx] = pre[i];
+        }
+    }
+};
+
 // contract modes M, M-1, ..., LO of phase PH (fastest first)
 template<typename T, typename PL, int PH, int M>
-__device__ __forceinline__ void contract_phase(T (&v)[PL::R], T const *const *__restrict__ A_list, int stride)
+__device__ __forceinline__ void contract_phase(T (&v)[PL::R], T const *const *__restrict__ A_list, int stride,
+                                               T const *coef)
 {
     if constexpr (M >= PL::lo(PH))
     {
-        contract_mode<T, PL::kN, PL::R, PL::reg_stride(M - PL::lo(PH))>(v, A_list[M], stride);
-        contract_phase<T, PL, PH, M - 1>(v, A_list, stride);
+        if constexpr (coef_in_lds<PL>())
+            contract_mode_lds<T, PL::kN, PL::R, PL::reg_stride(M - PL::lo(PH))>(v, coef + M * PL::kN * PL::kN);
+        else
+            contract_mode<T, PL::kN, PL::R, PL::reg_stride(M - PL::lo(PH))>(v, A_list[M], stride);
+        contract_phase<T, PL, PH, M - 1>(v, A_list, stride, coef);
     }
 }
 
@@ -213,16 +285,16 @@ __device__ __forceinline__ void load_input(T const *src_generic, T (&pre)[PL::R]
 
 template<typename T, typename PL, int PH>
 __device__ __forceinline__ void run_phases(T *lds, T (&v)[PL::R], T const *const *__restrict__ A_list, int stride,
-                                           int t)
+                                           T const *coef, int t)
 {
-    contract_phase<T, PL, PH, PL::c_hi(PH) - 1>(v, A_list, stride);
+    contract_phase<T, PL, PH, PL::c_hi(PH) - 1>(v, A_list, stride, coef);
     if constexpr (PH + 1 < PL::NPHASE)
     {
         lds_barrier();  // every thread has read phase PH
         Lds<T, PL>::template write<PH>(lds, v, t);
         lds_barrier();
         Lds<T, PL>::template read<PH + 1>(lds, v, t);
-        run_phases<T, PL, PH + 1>(lds, v, A_list, stride, t);
+        run_phases<T, PL, PH + 1>(lds, v, A_list, stride, coef, t);
     }
 }
 
@@ -248,13 +320,24 @@ __device__ __forceinline__ int xcd_run_index()
     return (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
 }
 
+// Waves per SIMD the register allocator must allow (the kernel's VGPR budget is 512 / waves): chosen so
+// that an extra workgroup fits per CU where the plan is a few registers over a threshold.
 template<typename T, typename PL>
-__global__ void __launch_bounds__(PL::P)
+constexpr int min_waves_per_simd()
+{
+    return (PL::kD == 6 && PL::kN == 4 && sizeof(T) == 8) ? 3 : (PL::kD == 5 && PL::kN == 4 && sizeof(T) == 8) ? 3 : 1;
+}
+
+template<typename T, typename PL>
+__global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(min_waves_per_simd<T, PL>())))
     kron_reg_kernel(int nb, int epw, T const *const *__restrict__ mats, int stride, T const *const *__restrict__ in,
                     T *const *__restrict__ out)
 {
     static_assert(PL::NPHASE >= 1, "");
-    __shared__ __attribute__((aligned(16))) T lds[PL::SIZE];
+    constexpr bool kCoefLds = coef_in_lds<PL>();
+    constexpr int kCoefSlots = kCoefLds ? Coef<T, PL>::kCount : 0;
+    __shared__ __attribute__((aligned(16))) T lds[PL::SIZE + kCoefSlots];
+    T *coef = lds + PL::SIZE;
     int const t = threadIdx.x;
     int const e0 = xcd_run_index() * epw;
     int const e1 = (e0 + epw < nb) ? e0 + epw : nb;
@@ -264,18 +347,26 @@ __global__ void __launch_bounds__(PL::P)
 #pragma unroll
     for (int j = 0; j < PL::R; ++j) acc[j] = T(0);
     T pre[PL::R];
+    T pre_coef[kCoefLds ? Coef<T, PL>::kPerThread : 1];
     T v[PL::R];
     load_input<T, PL>(in[e0], pre, t);
+    if constexpr (kCoefLds) Coef<T, PL>::load(mats + static_cast<size_t>(e0) * PL::kD, stride, pre_coef, t);
     T *cur = out[e0];
 
     for (int e = e0; e < e1; ++e)
     {
         lds_barrier();  // LDS free: the previous entry's last phase has been read by everyone
         Lds<T, PL>::store_input(lds, pre, t);
+        if constexpr (kCoefLds) Coef<T, PL>::store(coef, pre_coef, t);
         lds_barrier();
-        if (e + 1 < e1) load_input<T, PL>(in[e + 1], pre, t);
+        if (e + 1 < e1)
+        {
+            load_input<T, PL>(in[e + 1], pre, t);
+            if constexpr (kCoefLds)
+                Coef<T, PL>::load(mats + static_cast<size_t>(e + 1) * PL::kD, stride, pre_coef, t);
+        }
         Lds<T, PL>::template read<0>(lds, v, t);
-        run_phases<T, PL, 0>(lds, v, mats + static_cast<size_t>(e) * PL::kD, stride, t);
+        run_phases<T, PL, 0>(lds, v, mats + static_cast<size_t>(e) * PL::kD, stride, coef, t);
 
         T *dst = out[e];
         if (dst != cur)

@@ -146,66 +146,15 @@ hipError_t validate(int d, int n, int stride, int nb, Shape &s)
     return hipSuccess;
 }
 
-enum class Path
-{
-    Reg,
-    FiberLds,
-    FiberGlobal,
-    Unsupported
-};
-
 // Register-resident plans (d, n) -> RHO. Threads per workgroup P = n^(d-RHO) must be a multiple of 64
-// and at most 1024; n^d elements per entry stay in LDS between phases.
+// and at most 1024; the n^d elements of an entry stay in LDS between phases.
 constexpr int reg_rho(int d, int n)
 {
     return (d == 6 && n == 4) ? 2 : (d == 5 && n == 4) ? 2 : (d == 4 && n == 4) ? 1
          : (d == 4 && n == 8) ? 1 : (d == 3 && n == 8) ? 1 : 0;
 }
 
-template<typename T>
-Path choose_path(Shape const &s)
-{
-    if (reg_rho(s.d, s.n) > 0) return Path::Reg;
-    if (s.n > kMaxFiber) return Path::Unsupported;
-    long const coef_bytes = (s.n <= 8) ? 0 : static_cast<long>(s.n) * s.n * sizeof(T);
-    if (s.size * static_cast<long>(sizeof(T)) + coef_bytes <= kLdsBudget) return Path::FiberLds;
-    return Path::FiberGlobal;
-}
-
-template<typename T, int N, bool InLds>
-hipError_t launch_fiber(Shape const &s, T const *const *mats, T const *const *in, T *const *out, T *const *ws,
-                        hipStream_t stream)
-{
-    long const nfib = s.size / s.n;
-    int threads = static_cast<int>(((nfib + 63) / 64) * 64);
-    if (threads > kFiberThreads) threads = kFiberThreads;
-    size_t lds = 0;
-    if (InLds) lds += static_cast<size_t>(s.size) * sizeof(T);
-    if (N == 0) lds += static_cast<size_t>(s.n) * s.n * sizeof(T);
-    hipLaunchKernelGGL((kron_fiber_kernel<T, N, InLds>), dim3(s.nb), dim3(threads), lds, stream, s.d, s.n,
-                       static_cast<int>(s.size), mats, s.stride, in, out, ws);
-    return hipGetLastError();
-}
-
-template<typename T, bool InLds>
-hipError_t dispatch_fiber_n(Shape const &s, T const *const *mats, T const *const *in, T *const *out,
-                            T *const *ws, hipStream_t stream)
-{
-    switch (s.n)
-    {
-    case 1: return launch_fiber<T, 1, InLds>(s, mats, in, out, ws, stream);
-    case 2: return launch_fiber<T, 2, InLds>(s, mats, in, out, ws, stream);
-    case 3: return launch_fiber<T, 3, InLds>(s, mats, in, out, ws, stream);
-    case 4: return launch_fiber<T, 4, InLds>(s, mats, in, out, ws, stream);
-    case 5: return launch_fiber<T, 5, InLds>(s, mats, in, out, ws, stream);
-    case 6: return launch_fiber<T, 6, InLds>(s, mats, in, out, ws, stream);
-    case 7: return launch_fiber<T, 7, InLds>(s, mats, in, out, ws, stream);
-    case 8: return launch_fiber<T, 8, InLds>(s, mats, in, out, ws, stream);
-    default: return launch_fiber<T, 0, InLds>(s, mats, in, out, ws, stream);
-    }
-}
-
-// Number of compute units of the current device (cached per device; the value never changes).
+// Number of compute units of the current device (cached per device).
 int device_cu_count()
 {
     static std::atomic<int> cache[64];
@@ -220,70 +169,163 @@ int device_cu_count()
     return v;
 }
 
-// Resident workgroups per CU of a kernel (cached per kernel; the occupancy does not depend on the device
-// beyond the architecture this library is built for).
+// Resident workgroups per CU of one kernel instantiation (cached; it depends only on the code object).
 template<typename Kernel>
-int resident_per_cu(Kernel kernel, int block)
+int resident_per_cu(Kernel kernel, int block, size_t dyn_lds)
 {
     static std::atomic<int> cache{0};
     int v = cache.load(std::memory_order_relaxed);
     if (v == 0)
     {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, block, 0) != hipSuccess || v <= 0) v = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, block, dyn_lds) != hipSuccess || v <= 0) v = 1;
         cache.store(v, std::memory_order_relaxed);
     }
     return v;
 }
 
-// The register-resident sweep is launched as exactly one resident round: grid = CUs x resident
-// workgroups per CU (a multiple of 8, so every XCD gets the same number of runs), and each workgroup walks
-// a run of ceil(nb / grid) consecutive entries.
+struct Launch
+{
+    int family = 0;
+    long grid = 0;
+    int block = 0;
+    int epw = 1;
+    size_t lds = 0;
+    int resident = 0;
+    bool workspace = false;
+};
+
+// ---- register family --------------------------------------------------------------------------------
+// Launched as one resident round: grid = CUs x workgroups per CU (a multiple of 8, so every XCD gets the
+// same number of runs); each workgroup walks ceil(nb / grid) consecutive entries.
 template<typename T, int D, int N>
-hipError_t launch_reg(Shape const &s, T const *const *mats, T const *const *in, T *const *out, hipStream_t stream)
+hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T const *const *mats,
+                      T const *const *in, T *const *out, hipStream_t stream, bool run)
 {
     using PL = kronplan::Plan<D, N, reg_rho(D, N)>;
     static_assert(PL::P % 64 == 0 && PL::P <= 1024, "plan must fill whole wavefronts");
     auto kernel = kronreg::kron_reg_kernel<T, PL>;
-    long const slots = static_cast<long>(device_cu_count()) * resident_per_cu(kernel, PL::P);
-    int epw = static_cast<int>((s.nb + slots - 1) / slots);
+    L.family = KRONMULT_FAMILY_REGISTER;
+    L.block = PL::P;
+    L.lds = sizeof(T) * (PL::SIZE + (kronreg::coef_in_lds<PL>() ? kronreg::Coef<T, PL>::kCount : 0));
+    L.resident = resident_per_cu(kernel, PL::P, 0);
+    int const per_cu = tu.workgroups_per_cu > 0 ? tu.workgroups_per_cu : L.resident;
+    long const slots = static_cast<long>(device_cu_count()) * per_cu;
+    int epw = tu.entries_per_workgroup > 0 ? tu.entries_per_workgroup : static_cast<int>((s.nb + slots - 1) / slots);
     if (epw < 1) epw = 1;
-    long runs = (s.nb + epw - 1) / epw;
-    runs = (runs + 7) / 8 * 8;  // whole rounds over the 8 XCDs; surplus runs exit at once
-    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(runs)), dim3(PL::P), 0, stream, s.nb, epw, mats, s.stride,
-                       in, out);
+    L.epw = epw;
+    long const runs = (s.nb + epw - 1) / epw;
+    L.grid = (runs + 7) / 8 * 8;  // surplus runs exit at once
+    if (!run) return hipSuccess;
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(L.grid)), dim3(PL::P), 0, stream, s.nb, epw, mats,
+                       s.stride, in, out);
     return hipGetLastError();
 }
 
 template<typename T>
-hipError_t dispatch_reg(Shape const &s, T const *const *mats, T const *const *in, T *const *out, hipStream_t stream)
+hipError_t reg_dispatch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T const *const *mats,
+                        T const *const *in, T *const *out, hipStream_t stream, bool run)
 {
-    if (s.d == 6 && s.n == 4) return launch_reg<T, 6, 4>(s, mats, in, out, stream);
-    if (s.d == 5 && s.n == 4) return launch_reg<T, 5, 4>(s, mats, in, out, stream);
-    if (s.d == 4 && s.n == 4) return launch_reg<T, 4, 4>(s, mats, in, out, stream);
-    if (s.d == 4 && s.n == 8) return launch_reg<T, 4, 8>(s, mats, in, out, stream);
-    if (s.d == 3 && s.n == 8) return launch_reg<T, 3, 8>(s, mats, in, out, stream);
+    if (s.d == 6 && s.n == 4) return reg_launch<T, 6, 4>(s, tu, L, mats, in, out, stream, run);
+    if (s.d == 5 && s.n == 4) return reg_launch<T, 5, 4>(s, tu, L, mats, in, out, stream, run);
+    if (s.d == 4 && s.n == 4) return reg_launch<T, 4, 4>(s, tu, L, mats, in, out, stream, run);
+    if (s.d == 4 && s.n == 8) return reg_launch<T, 4, 8>(s, tu, L, mats, in, out, stream, run);
+    if (s.d == 3 && s.n == 8) return reg_launch<T, 3, 8>(s, tu, L, mats, in, out, stream, run);
     return hipErrorInvalidValue;
+}
+
+// ---- fiber family -------------------------------------------------------------------------------------
+template<typename T>
+bool fiber_in_lds(Shape const &s)
+{
+    long const coef_bytes = (s.n <= 8) ? 0 : static_cast<long>(s.n) * s.n * sizeof(T);
+    return s.size * static_cast<long>(sizeof(T)) + coef_bytes <= kLdsBudget;
+}
+
+template<typename T, int N, bool InLds>
+hipError_t fiber_launch(Shape const &s, Launch &L, T const *const *mats, T const *const *in, T *const *out,
+                        T *const *ws, hipStream_t stream, bool run)
+{
+    long const nfib = s.size / s.n;
+    int threads = static_cast<int>(((nfib + 63) / 64) * 64);
+    if (threads > kFiberThreads) threads = kFiberThreads;
+    size_t lds = 0;
+    if (InLds) lds += static_cast<size_t>(s.size) * sizeof(T);
+    if (N == 0) lds += static_cast<size_t>(s.n) * s.n * sizeof(T);
+    auto kernel = kron_fiber_kernel<T, N, InLds>;
+    L.family = KRONMULT_FAMILY_FIBER;
+    L.grid = s.nb;
+    L.block = threads;
+    L.epw = 1;
+    L.lds = lds;
+    L.workspace = !InLds;
+    L.resident = resident_per_cu(kernel, threads, lds);
+    if (!run) return hipSuccess;
+    hipLaunchKernelGGL(kernel, dim3(s.nb), dim3(threads), lds, stream, s.d, s.n, static_cast<int>(s.size), mats,
+                       s.stride, in, out, ws);
+    return hipGetLastError();
+}
+
+template<typename T, bool InLds>
+hipError_t fiber_dispatch_n(Shape const &s, Launch &L, T const *const *mats, T const *const *in, T *const *out,
+                            T *const *ws, hipStream_t stream, bool run)
+{
+    switch (s.n)
+    {
+    case 1: return fiber_launch<T, 1, InLds>(s, L, mats, in, out, ws, stream, run);
+    case 2: return fiber_launch<T, 2, InLds>(s, L, mats, in, out, ws, stream, run);
+    case 3: return fiber_launch<T, 3, InLds>(s, L, mats, in, out, ws, stream, run);
+    case 4: return fiber_launch<T, 4, InLds>(s, L, mats, in, out, ws, stream, run);
+    case 5: return fiber_launch<T, 5, InLds>(s, L, mats, in, out, ws, stream, run);
+    case 6: return fiber_launch<T, 6, InLds>(s, L, mats, in, out, ws, stream, run);
+    case 7: return fiber_launch<T, 7, InLds>(s, L, mats, in, out, ws, stream, run);
+    case 8: return fiber_launch<T, 8, InLds>(s, L, mats, in, out, ws, stream, run);
+    default: return fiber_launch<T, 0, InLds>(s, L, mats, in, out, ws, stream, run);
+    }
+}
+
+template<typename T>
+hipError_t fiber_dispatch(Shape const &s, Launch &L, T const *const *mats, T const *const *in, T *const *out,
+                          T *const *ws, hipStream_t stream, bool run)
+{
+    if (s.n > kMaxFiber) return hipErrorInvalidValue;
+    if (fiber_in_lds<T>(s)) return fiber_dispatch_n<T, true>(s, L, mats, in, out, ws, stream, run);
+    if (run && ws == nullptr) return hipErrorInvalidValue;
+    return fiber_dispatch_n<T, false>(s, L, mats, in, out, ws, stream, run);
+}
+
+// ---- entry --------------------------------------------------------------------------------------------
+int choose_family(Shape const &s, kronmult_tuning_t const &tu)
+{
+    bool const reg_ok = reg_rho(s.d, s.n) > 0;
+    if (tu.family == KRONMULT_FAMILY_REGISTER) return reg_ok ? KRONMULT_FAMILY_REGISTER : -1;
+    if (tu.family == KRONMULT_FAMILY_FIBER) return s.n <= kMaxFiber ? KRONMULT_FAMILY_FIBER : -1;
+    if (tu.family != KRONMULT_FAMILY_AUTO) return -1;
+    if (reg_ok) return KRONMULT_FAMILY_REGISTER;
+    return s.n <= kMaxFiber ? KRONMULT_FAMILY_FIBER : -1;
 }
 
 template<typename T>
 hipError_t kronmult_launch(int d, int n, T const *const *mats, int stride, T *const *in, T *const *out,
-                           T *const *ws, int nb, hipStream_t stream)
+                           T *const *ws, int nb, kronmult_tuning_t const *tuning, hipStream_t stream, Launch *info)
 {
     Shape s{};
     hipError_t err = validate(d, n, stride, nb, s);
     if (err != hipSuccess) return err;
-    if (nb == 0) return hipSuccess;
-    if (mats == nullptr || in == nullptr || out == nullptr) return hipErrorInvalidValue;
-    T const *const *cin = reinterpret_cast<T const *const *>(in);
-    switch (choose_path<T>(s))
+    kronmult_tuning_t const tu = tuning ? *tuning : kronmult_tuning_t{0, 0, 0};
+    if (tu.workgroups_per_cu < 0 || tu.entries_per_workgroup < 0) return hipErrorInvalidValue;
+    int const family = choose_family(s, tu);
+    if (family < 0) return hipErrorInvalidValue;
+    bool const run = info == nullptr;
+    if (run)
     {
-    case Path::Reg: return dispatch_reg<T>(s, mats, cin, out, stream);
-    case Path::FiberLds: return dispatch_fiber_n<T, true>(s, mats, cin, out, ws, stream);
-    case Path::FiberGlobal:
-        if (ws == nullptr) return hipErrorInvalidValue;
-        return dispatch_fiber_n<T, false>(s, mats, cin, out, ws, stream);
-    default: return hipErrorInvalidValue;
+        if (nb == 0) return hipSuccess;
+        if (mats == nullptr || in == nullptr || out == nullptr) return hipErrorInvalidValue;
     }
+    Launch local;
+    Launch &L = info ? *info : local;
+    T const *const *cin = reinterpret_cast<T const *const *>(in);
+    if (family == KRONMULT_FAMILY_REGISTER) return reg_dispatch<T>(s, tu, L, mats, cin, out, stream, run);
+    return fiber_dispatch<T>(s, L, mats, cin, out, ws, stream, run);
 }
 
 template<typename T>
@@ -292,7 +334,7 @@ int kronmult_sync(int d, int n, T const *const *mats, int stride, T **in, T **ou
     Shape s{};
     hipError_t err = validate(d, n, stride, nb, s);
     if (err != hipSuccess || nb == 0) return static_cast<int>(err);
-    err = kronmult_launch<T>(d, n, mats, stride, in, out, ws, nb, nullptr);
+    err = kronmult_launch<T>(d, n, mats, stride, in, out, ws, nb, nullptr, nullptr, nullptr);
     if (err != hipSuccess) return static_cast<int>(err);
     return static_cast<int>(hipDeviceSynchronize());
 }
@@ -303,11 +345,12 @@ char const *kernel_name(int d, int n, int nb)
     Shape s{};
     if (validate(d, n, n, nb < 0 ? 0 : nb, s) != hipSuccess) return "invalid";
     bool const f64 = sizeof(T) == 8;
-    switch (choose_path<T>(s))
+    switch (choose_family(s, kronmult_tuning_t{0, 0, 0}))
     {
-    case Path::Reg: return f64 ? "reg_sweep_f64" : "reg_sweep_f32";
-    case Path::FiberLds: return f64 ? "fiber_lds_f64" : "fiber_lds_f32";
-    case Path::FiberGlobal: return f64 ? "fiber_global_f64" : "fiber_global_f32";
+    case KRONMULT_FAMILY_REGISTER: return f64 ? "reg_sweep_f64" : "reg_sweep_f32";
+    case KRONMULT_FAMILY_FIBER:
+        if (fiber_in_lds<T>(s)) return f64 ? "fiber_lds_f64" : "fiber_lds_f32";
+        return f64 ? "fiber_global_f64" : "fiber_global_f32";
     default: return "unsupported";
     }
 }
@@ -337,25 +380,63 @@ int kronmult_batched_f64_async(int matrix_count, int matrix_size, double const *
                                int matrix_stride, double **input_batched, double **output_batched,
                                double **workspace_batched, int nb_batch, void *stream)
 {
-    return static_cast<int>(kronmult_launch<double>(matrix_count, matrix_size, matrix_list_batched,
-                                                    matrix_stride, input_batched, output_batched,
-                                                    workspace_batched, nb_batch,
-                                                    static_cast<hipStream_t>(stream)));
+    return static_cast<int>(kronmult_launch<double>(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
+                                                    input_batched, output_batched, workspace_batched, nb_batch,
+                                                    nullptr, static_cast<hipStream_t>(stream), nullptr));
 }
 
 int kronmult_batched_f32_async(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
                                int matrix_stride, float **input_batched, float **output_batched,
                                float **workspace_batched, int nb_batch, void *stream)
 {
-    return static_cast<int>(kronmult_launch<float>(matrix_count, matrix_size, matrix_list_batched,
-                                                   matrix_stride, input_batched, output_batched,
-                                                   workspace_batched, nb_batch,
-                                                   static_cast<hipStream_t>(stream)));
+    return static_cast<int>(kronmult_launch<float>(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
+                                                   input_batched, output_batched, workspace_batched, nb_batch,
+                                                   nullptr, static_cast<hipStream_t>(stream), nullptr));
+}
+
+int kronmult_batched_f64_tuned(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                               int matrix_stride, double **input_batched, double **output_batched,
+                               double **workspace_batched, int nb_batch, kronmult_tuning_t const *tuning,
+                               void *stream)
+{
+    return static_cast<int>(kronmult_launch<double>(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
+                                                    input_batched, output_batched, workspace_batched, nb_batch,
+                                                    tuning, static_cast<hipStream_t>(stream), nullptr));
+}
+
+int kronmult_batched_f32_tuned(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
+                               int matrix_stride, float **input_batched, float **output_batched,
+                               float **workspace_batched, int nb_batch, kronmult_tuning_t const *tuning,
+                               void *stream)
+{
+    return static_cast<int>(kronmult_launch<float>(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
+                                                   input_batched, output_batched, workspace_batched, nb_batch,
+                                                   tuning, static_cast<hipStream_t>(stream), nullptr));
+}
+
+int kronmult_hip_launch_info(int matrix_count, int matrix_size, int element_size, int nb_batch,
+                             kronmult_tuning_t const *tuning, kronmult_launch_info_t *info)
+{
+    if (info == nullptr) return static_cast<int>(hipErrorInvalidValue);
+    Launch L;
+    hipError_t err;
+    if (element_size == 8)
+        err = kronmult_launch<double>(matrix_count, matrix_size, nullptr, matrix_size, nullptr, nullptr, nullptr,
+                                      nb_batch, tuning, nullptr, &L);
+    else if (element_size == 4)
+        err = kronmult_launch<float>(matrix_count, matrix_size, nullptr, matrix_size, nullptr, nullptr, nullptr,
+                                     nb_batch, tuning, nullptr, &L);
+    else
+        err = hipErrorInvalidValue;
+    if (err != hipSuccess) return static_cast<int>(err);
+    *info = kronmult_launch_info_t{L.family, static_cast<int>(L.grid), L.block, L.epw, static_cast<int>(L.lds),
+                                   L.resident, L.workspace ? 1 : 0};
+    return 0;
 }
 
 char const *kronmult_hip_version(void)
 {
-    return "kronmult_hip 0.1 (gfx950)";
+    return "kronmult_hip 0.2 (gfx950)";
 }
 
 char const *kronmult_hip_kernel_name(int matrix_count, int matrix_size, int element_size, int nb_batch)

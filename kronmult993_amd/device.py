@@ -61,10 +61,14 @@ class DeviceBatch:
     def reset_output(self) -> None:
         self.out.copy_(self._out_init)
 
-    def run(self, sync: bool = True, stream=None) -> None:
-        """One call of the library over the whole batch."""
+    def run(self, sync: bool = True, stream=None, tuning=None) -> None:
+        """One call of the library over the whole batch (`tuning`: explicit launch shape, stream-ordered)."""
         args = (self.d, self.n, self.mat_ptrs, self.stride, self.in_ptrs, self.out_ptrs, self.ws_ptrs, self.nb)
-        if sync:
+        if tuning is not None:
+            api.kronmult_batched_tuned(*args, dtype=self.dtype_name, tuning=tuning, stream=stream)
+            if sync:
+                _torch().cuda.synchronize(self.device)
+        elif sync:
             api.kronmult_batched(*args, dtype=self.dtype_name)
         else:
             api.kronmult_batched_async(*args, dtype=self.dtype_name, stream=stream)
