@@ -57,24 +57,36 @@ struct Plan
 
 // LDS placement of canonical element e. The tensor is stored in 16-byte granules (G = 16 / sizeof(T)
 // elements); granule index g goes to g ^ x(g), where bit i (i < 4) of x is the parity of g & mask_i: a
-// linear (GF(2)) map of the granule bits into the four low ones, which pick the bank group.
+// linear (GF(2)) map of the granule bits into the four low ones, which pick the bank.
 //
-// gfx950's LDS was measured (SQ_LDS_BANK_CONFLICT on single-pattern probe kernels) to serve 8- and
-// 16-byte reads as 64 banks x 4 bytes, 256 bytes per cycle (ds_read_b64: 32 lanes per cycle,
-// ds_read_b128: 16 lanes), and 8- and 16-byte writes as 32 banks, 128 bytes per cycle (ds_write_b64: 16
-// lanes, ds_write_b128: 8 lanes). The masks below were found by search so that every LDS read and write
-// of each plan is conflict-free in that model; tests/cpp/plan_check.cpp verifies it for each plan.
-// Layout: mask_i in bits [16 i, 16 i + 16).
-constexpr unsigned long long swizzle_masks(int d, int n, int rho, int elem_bytes)
+// gfx950's LDS, measured with SQ_LDS_BANK_CONFLICT on probe kernels: banks are 16 bytes wide; reads are
+// served 256 bytes per cycle (16 banks), writes 128 bytes per cycle (8 banks); two lanes conflict when
+// they hit one bank in different 16-byte rows; 16-byte reads of a 32-lane half-wave can also conflict
+// across its two 16-lane groups. Each LDS round trip (a store followed by the loads that consume it) uses
+// its own map, which decouples the write and read patterns of different phases:
+//   stage 0: input store -> phase-0 read;   stage 1: phase-p write -> phase-(p+1) read.
+// The maps of the d = 6 / d = 5, n = 4 FP64 plans were measured conflict-free access by access on the
+// hardware; the others come from an exhaustive search under the bank model above. The CPU check
+// tests/cpp/plan_check.cpp verifies every plan against the model. Layout: mask_i in bits [16 i, 16 i + 16).
+constexpr unsigned long long swizzle_masks(int d, int n, int rho, int elem_bytes, int stage)
 {
-    return (d == 6 && n == 4 && rho == 2) ? (elem_bytes == 8 ? 0x00c0002000100008ull : 0x0020004000100008ull)
-         : (d == 5 && n == 4 && rho == 2) ? (elem_bytes == 8 ? 0x00c0002000100008ull : 0x0020004000100008ull)
-         : (d == 4 && n == 4 && rho == 1) ? (elem_bytes == 8 ? 0x0020001000080006ull : 0x0000001000080004ull)
-         : (d == 4 && n == 8 && rho == 1) ? (elem_bytes == 8 ? 0x0044002000100008ull : 0x0002002000100008ull)
-         : (d == 3 && n == 8 && rho == 1) ? (elem_bytes == 8 ? 0x0044002000100008ull : 0x0002002000100008ull)
-         : (d == 4 && n == 8 && rho == 2) ? (elem_bytes == 8 ? 0x0100008000400020ull : 0x0080004000200010ull)
-         : (d == 6 && n == 4 && rho == 3) ? (elem_bytes == 8 ? 0x0100008000400020ull : 0x0080004000200010ull)
-         : 0ull;
+    bool const f64 = elem_bytes == 8;
+    if ((d == 6 || d == 5) && n == 4 && rho == 2)
+    {
+        if (f64) return stage == 0 ? 0x0080004000200010ull : 0x00c0002000100008ull;
+        return stage == 0 ? 0x0020001000080004ull : 0x0000004000100008ull;
+    }
+    if (d == 4 && n == 4 && rho == 1)
+    {
+        if (f64) return stage == 0 ? 0x0010000800040002ull : 0x0020001000080002ull;
+        return stage == 0 ? 0x0ull : 0x0000001000080004ull;
+    }
+    if ((d == 4 || d == 3) && n == 8 && rho == 1)
+    {
+        if (f64) return stage == 0 ? 0x0020001000080004ull : 0x0040002000100008ull;
+        return stage == 0 ? 0x0010000800040002ull : 0x0000002000100008ull;
+    }
+    return 0ull;
 }
 
 template<unsigned long long MASKS, int G>
@@ -93,8 +105,8 @@ struct Swizzle
     }
 };
 
-template<typename PL, typename T>
-using PlanSwizzle = Swizzle<swizzle_masks(PL::kD, PL::kN, PL::kRho, static_cast<int>(sizeof(T))),
+template<typename PL, typename T, int STAGE>
+using PlanSwizzle = Swizzle<swizzle_masks(PL::kD, PL::kN, PL::kRho, static_cast<int>(sizeof(T)), STAGE),
                             static_cast<int>(16 / sizeof(T))>;
 
 } // namespace kronplan

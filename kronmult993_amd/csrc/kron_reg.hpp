@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "kron_plan.hpp"
 
@@ -180,7 +181,8 @@ __device__ __forceinline__ void contract_phase(T (&v)[PL::R], T const *const *__
 template<typename T, typename PL>
 struct Lds
 {
-    using SW = PlanSwizzle<PL, T>;
+    using SW_IN = PlanSwizzle<PL, T, 0>;  // input store -> phase-0 read
+    using SW_EX = PlanSwizzle<PL, T, 1>;  // phase-p write -> phase-(p+1) read
     static constexpr int G = Granule<T>::G;
     static constexpr bool kVec = PL::R >= G;
     using V = typename Granule<T>::type;
@@ -196,19 +198,20 @@ struct Lds
                 V g;
 #pragma unroll
                 for (int i = 0; i < G; ++i) g[i] = pre[c * G + i];
-                *reinterpret_cast<V *>(lds + SW::phys((c * PL::P + t) * G)) = g;
+                *reinterpret_cast<V *>(lds + SW_IN::phys((c * PL::P + t) * G)) = g;
             }
         }
         else
         {
 #pragma unroll
-            for (int c = 0; c < PL::R; ++c) lds[SW::phys(c * PL::P + t)] = pre[c];
+            for (int c = 0; c < PL::R; ++c) lds[SW_IN::phys(c * PL::P + t)] = pre[c];
         }
     }
 
     template<int PH>
     static __device__ __forceinline__ void read(T const *lds, T (&v)[PL::R], int t)
     {
+        using SW = typename std::conditional<PH == 0, SW_IN, SW_EX>::type;
         if constexpr (PH == 0 && kVec)
         {
 #pragma unroll
@@ -237,13 +240,13 @@ struct Lds
                 V g;
 #pragma unroll
                 for (int i = 0; i < G; ++i) g[i] = v[q * G + i];
-                *reinterpret_cast<V *>(lds + SW::phys(PL::elem(0, t, q * G))) = g;
+                *reinterpret_cast<V *>(lds + SW_EX::phys(PL::elem(0, t, q * G))) = g;
             }
         }
         else
         {
 #pragma unroll
-            for (int j = 0; j < PL::R; ++j) lds[SW::phys(PL::elem(PH, t, j))] = v[j];
+            for (int j = 0; j < PL::R; ++j) lds[SW_EX::phys(PL::elem(PH, t, j))] = v[j];
         }
     }
 };
