@@ -6,11 +6,18 @@ d=6, n=4 ASGarD-like operator apply in FP64 (8192 entries: 128 output elements x
 64x64 coefficient matrices whose 4x4 blocks are the factors, stride 64). Inputs are resident in HBM before
 the timed region; the outputs accumulate across steps as the operation specifies.
 
+Timing: the K steps are captured once as a HIP graph of K launches on the bench stream and replayed; the
+timed region (bracketed by device synchronisation) is one replay, so `ms_per_step` is steady-state
+throughput without host launch gaps. A separate eager pass records HIP events around every launch on the
+same stream; their mean is the kernel duration used for the roofline.
+
 Printed: one JSON line with GFLOP/s (value), the roofline of the kernel (contract bytes per launch over the
-average kernel duration, measured with HIP events on the launch stream, against 8 TB/s HBM; and FP64 flops
-against 78.6 TFLOP/s), and the reference's CPU implementation timed on this host.
+kernel duration against 8 TB/s HBM; flops against the FP64 / FP32 peak), the HBM traffic per launch
+measured with rocprofv3 counters (profiles/traffic.json), and the reference's CPU implementation timed on
+this host.
 
     python bench.py [--steps K] [--warmup W] [--config c2] [--dtype f64] [--cpu-baseline auto|off]
+                    [--calibrate-copy]
 """
 import argparse
 import json
@@ -81,14 +88,18 @@ def cpu_baseline(hb, runs: int):
                        f"copies, 1 warm-up + {runs} runs, median {med * 1e3:.1f} ms, min {min(times) * 1e3:.1f} ms")}
 
 
-def load_traffic(key: str):
+def load_traffic(key: str, kernel: str):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/traffic.json), if
+    recorded for this workload and kernel family."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         table = json.load(f)
     entry = table.get(key)
-    return None if entry is None else entry.get("bytes_per_launch")
+    if entry is None or entry.get("kernel") != kernel:
+        return None
+    return entry.get("bytes_per_launch")
 
 
 def main() -> int:
@@ -100,11 +111,13 @@ def main() -> int:
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
     ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after timing, copy a 256 MiB device buffer (reference kernel for byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        log("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
-            "(the operator would shard by output rows with no exchange; see DESIGN.md).")
+        log("bench.py measures one GPU: this library runs the batch on a single MI355X and has no "
+            "multi-GPU path (the operator would shard by output rows, with no exchange between GPUs).")
         return 2
 
     import numpy as np
@@ -126,26 +139,40 @@ def main() -> int:
         db.run(sync=False, stream=stream)
     torch.cuda.synchronize(dev)
 
+    # eager pass: HIP events around every launch on the launch stream -> kernel duration
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize(dev)
-    wall0 = time.perf_counter()
-    t_start.record(stream)
     for i in range(args.steps):
         starts[i].record(stream)
         db.run(sync=False, stream=stream)
         ends[i].record(stream)
+    torch.cuda.synchronize(dev)
+    kernel_ms = [a.elapsed_time(b) for a, b in zip(starts, ends)]
+    k_avg = statistics.mean(kernel_ms)
+
+    # timed region: one replay of a graph holding the K launches
+    graph = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream(device=dev)
+    cap.wait_stream(stream)
+    with torch.cuda.stream(cap):
+        with torch.cuda.graph(graph, stream=cap):
+            for _ in range(args.steps):
+                db.run(sync=False, stream=cap)
+    torch.cuda.synchronize(dev)
+    graph.replay()  # warm replay
+    torch.cuda.synchronize(dev)
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    graph.replay()
     t_end.record(stream)
     torch.cuda.synchronize(dev)
     wall = time.perf_counter() - wall0
     total_ms = t_start.elapsed_time(t_end)
-    kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
     ms_per_step = total_ms / args.steps
-    k_avg = statistics.mean(kernel_ms)
-    log(f"[bench] {args.steps} steps: {ms_per_step * 1e3:.2f} us/step (events), kernel avg {k_avg * 1e3:.2f} us, "
-        f"min {min(kernel_ms) * 1e3:.2f} us, wall {wall * 1e6 / args.steps:.2f} us/step")
+    log(f"[bench] {args.steps} steps: {ms_per_step * 1e3:.2f} us/step (graph replay), kernel avg "
+        f"{k_avg * 1e3:.2f} us (eager, min {min(kernel_ms) * 1e3:.2f}), wall {wall * 1e6 / args.steps:.2f} us/step")
 
     flops = hb.flops()
     contract = hb.contract_bytes()
@@ -172,7 +199,7 @@ def main() -> int:
                    "kernel": kernel},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(f"{args.config}_{args.dtype}")},
+                     "traffic": load_traffic(f"{args.config}_{args.dtype}", kernel)},
         "roofline_compute": {"bound": "fp64" if args.dtype == "f64" else "fp32", "achieved": round(achieved_tf, 3),
                              "peak": peak_tf, "unit": "TFLOP/s", "frac": round(achieved_tf / peak_tf, 4)},
         "kernel_us": {"avg": round(k_avg * 1e3, 3), "min": round(min(kernel_ms) * 1e3, 3),
@@ -182,6 +209,13 @@ def main() -> int:
     }
     if args.cpu_baseline == "auto":
         result["cpu_baseline"] = cpu_baseline(hb, args.cpu_runs)
+    if args.calibrate_copy:
+        src = torch.ones(256 * 1024 * 1024 // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize(dev)
+        dst.copy_(src)
+        torch.cuda.synchronize(dev)
+        log("[bench] calibration copy: 268435456 bytes read, 268435456 bytes written")
     print(json.dumps(result), flush=True)
     return 0
 

@@ -195,8 +195,14 @@ struct Launch
 };
 
 // ---- register family --------------------------------------------------------------------------------
-// Launched as one resident round: grid = CUs x workgroups per CU (a multiple of 8, so every XCD gets the
-// same number of runs); each workgroup walks ceil(nb / grid) consecutive entries.
+// grid = CUs x workgroups per CU (a multiple of 8, so every XCD gets the same number of runs); each
+// workgroup walks ceil(nb / grid) consecutive entries. The default number of workgroups per CU was
+// measured on MI355X (launch-shape sweeps of the d=6 n=4, d=3 n=8 and d=4 n=8 batches): about 16
+// wavefronts per CU for 4- and 1-wavefront workgroups, a single 8-wavefront workgroup.
+constexpr int default_workgroups_per_cu(int threads)
+{
+    return threads >= 512 ? 1 : threads >= 256 ? 4 : threads >= 128 ? 4 : 8;
+}
 template<typename T, int D, int N>
 hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T const *const *mats,
                       T const *const *in, T *const *out, hipStream_t stream, bool run)
@@ -208,7 +214,7 @@ hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T 
     L.block = PL::P;
     L.lds = sizeof(T) * (PL::SIZE + (kronreg::coef_in_lds<PL>() ? kronreg::Coef<T, PL>::kCount : 0));
     L.resident = resident_per_cu(kernel, PL::P, 0);
-    int const per_cu = tu.workgroups_per_cu > 0 ? tu.workgroups_per_cu : L.resident;
+    int const per_cu = tu.workgroups_per_cu > 0 ? tu.workgroups_per_cu : default_workgroups_per_cu(PL::P);
     long const slots = static_cast<long>(device_cu_count()) * per_cu;
     int epw = tu.entries_per_workgroup > 0 ? tu.entries_per_workgroup : static_cast<int>((s.nb + slots - 1) / slots);
     if (epw < 1) epw = 1;
