@@ -97,6 +97,53 @@ int kronmult_hip_launch_info(int matrix_count, int matrix_size, int element_size
                              kronmult_tuning_t const *tuning, kronmult_launch_info_t *info);
 
 /*
+ * Adjoint (no counterpart in the reference, which has no transposed or gradient API). For the cotangent
+ * g_o = grad_output_batched-addressed vectors on the outputs of the forward call above (same pointer
+ * arrays, so entries that share an output read the same g), accumulates for every entry k:
+ *   grad_input_batched[k]        += kron(A_{k,0}^T, ..., A_{k,d-1}^T) · g_{o(k)}
+ *   grad_matrix_batched[k*d + t] += dL/dA_{k,t}   (n x n, column-major, leading dimension matrix_stride)
+ * with L = sum_o <g_o, output_o>. Either output array may be NULL (that gradient is skipped). Aliased
+ * gradient pointers (entries that share a factor or an input) receive the sum of the contributions
+ * (hardware atomic adds, so the order of that sum is not fixed). input_batched is read only when the
+ * factor gradients are requested. Supports n <= 8.
+ *
+ * intermediates selects how the forward sweep's intermediate vectors are obtained:
+ *   KRONMULT_ADJOINT_RECOMPUTE  rebuilt from the input for every factor (two n^d vectors of scratch);
+ *   KRONMULT_ADJOINT_STORE      computed once and kept (d vectors of scratch);
+ *   KRONMULT_ADJOINT_AUTO       STORE when its scratch fits in on-chip memory, else RECOMPUTE.
+ * Scratch is on-chip when it fits; otherwise workspace_batched[k] must point to
+ * kronmult_batched_adjoint_workspace(d, n, sizeof(T), intermediates) elements per entry (0 = not read).
+ */
+#define KRONMULT_ADJOINT_AUTO 0
+#define KRONMULT_ADJOINT_RECOMPUTE 1
+#define KRONMULT_ADJOINT_STORE 2
+
+int kronmult_batched_adjoint_f64(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                                 int matrix_stride, double const *const *input_batched,
+                                 double const *const *grad_output_batched, double **grad_input_batched,
+                                 double **grad_matrix_batched, double **workspace_batched, int nb_batch,
+                                 int intermediates);
+int kronmult_batched_adjoint_f32(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
+                                 int matrix_stride, float const *const *input_batched,
+                                 float const *const *grad_output_batched, float **grad_input_batched,
+                                 float **grad_matrix_batched, float **workspace_batched, int nb_batch,
+                                 int intermediates);
+int kronmult_batched_adjoint_f64_async(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                                       int matrix_stride, double const *const *input_batched,
+                                       double const *const *grad_output_batched, double **grad_input_batched,
+                                       double **grad_matrix_batched, double **workspace_batched, int nb_batch,
+                                       int intermediates, void *stream);
+int kronmult_batched_adjoint_f32_async(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
+                                       int matrix_stride, float const *const *input_batched,
+                                       float const *const *grad_output_batched, float **grad_input_batched,
+                                       float **grad_matrix_batched, float **workspace_batched, int nb_batch,
+                                       int intermediates, void *stream);
+
+/* Workspace elements per entry the adjoint reads for (d, n, element size in bytes, intermediates) on the
+ * current device: 0 when its scratch fits on chip, -1 for unsupported arguments. */
+long long kronmult_batched_adjoint_workspace(int matrix_count, int matrix_size, int element_size, int intermediates);
+
+/*
  * Library version string and the name of the kernel the dispatcher would pick for (d, n, element size in
  * bytes, nb_batch); both point to static storage. No reference counterpart (diagnostics only).
  */

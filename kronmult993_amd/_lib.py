@@ -43,6 +43,11 @@ SIGNATURES = {
     "kronmult_batched_f64_tuned": (_I, [_I, _I, _P, _I, _P, _P, _P, _I, ctypes.POINTER(Tuning), _P]),
     "kronmult_batched_f32_tuned": (_I, [_I, _I, _P, _I, _P, _P, _P, _I, ctypes.POINTER(Tuning), _P]),
     "kronmult_hip_launch_info": (_I, [_I, _I, _I, _I, ctypes.POINTER(Tuning), ctypes.POINTER(LaunchInfo)]),
+    "kronmult_batched_adjoint_f64": (_I, [_I, _I, _P, _I, _P, _P, _P, _P, _P, _I, _I]),
+    "kronmult_batched_adjoint_f32": (_I, [_I, _I, _P, _I, _P, _P, _P, _P, _P, _I, _I]),
+    "kronmult_batched_adjoint_f64_async": (_I, [_I, _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _P]),
+    "kronmult_batched_adjoint_f32_async": (_I, [_I, _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _P]),
+    "kronmult_batched_adjoint_workspace": (ctypes.c_longlong, [_I, _I, _I, _I]),
     "kronmult_hip_version": (ctypes.c_char_p, []),
     "kronmult_hip_kernel_name": (ctypes.c_char_p, [_I, _I, _I, _I]),
 }

@@ -118,6 +118,42 @@ def launch_info(matrix_count: int, matrix_size: int, element_size: int, nb_batch
     return info.as_dict()
 
 
+ADJOINT_AUTO, ADJOINT_RECOMPUTE, ADJOINT_STORE = 0, 1, 2
+_INTERMEDIATES = {"auto": ADJOINT_AUTO, "recompute": ADJOINT_RECOMPUTE, "store": ADJOINT_STORE}
+
+
+def _intermediates(v) -> int:
+    return _INTERMEDIATES[v] if isinstance(v, str) else int(v)
+
+
+def kronmult_batched_adjoint(matrix_count: int, matrix_size: int, matrix_list_batched, matrix_stride: int,
+                             input_batched, grad_output_batched, grad_input_batched, grad_matrix_batched,
+                             workspace_batched, nb_batch: int, *, dtype, intermediates="auto", stream=None,
+                             sync: bool = True) -> None:
+    """Adjoint of kronmult_batched (no reference counterpart): accumulates K^T g into grad_input_batched[k]
+    and dL/dA into grad_matrix_batched[k*d+t] (either may be None). `intermediates`: "auto", "recompute"
+    or "store". Synchronous unless sync=False (then stream-ordered on `stream`)."""
+    sfx = _suffix(dtype)
+    args = [matrix_count, matrix_size, _addr(matrix_list_batched), matrix_stride, _addr(input_batched),
+            _addr(grad_output_batched), _addr(grad_input_batched), _addr(grad_matrix_batched),
+            _addr(workspace_batched), nb_batch, _intermediates(intermediates)]
+    if sync:
+        rc = getattr(lib(), "kronmult_batched_adjoint_" + sfx)(*args)
+    else:
+        rc = getattr(lib(), "kronmult_batched_adjoint_" + sfx + "_async")(*args, _stream_handle(stream))
+    if rc != HIP_SUCCESS:
+        raise KronmultError(rc, "kronmult_batched_adjoint")
+
+
+def adjoint_workspace(matrix_count: int, matrix_size: int, element_size: int, intermediates="auto") -> int:
+    """Workspace elements per entry the adjoint reads (0: none); raises for unsupported arguments."""
+    v = lib().kronmult_batched_adjoint_workspace(matrix_count, matrix_size, element_size,
+                                                  _intermediates(intermediates))
+    if v < 0:
+        raise KronmultError(HIP_ERROR_INVALID_VALUE, "kronmult_batched_adjoint_workspace")
+    return int(v)
+
+
 def kernel_name(matrix_count: int, matrix_size: int, element_size: int, nb_batch: int) -> str:
     return lib().kronmult_hip_kernel_name(matrix_count, matrix_size, element_size, nb_batch).decode()
 

@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstdio>
 
+#include "kron_adjoint.hpp"
 #include "kron_reg.hpp"
 #include "kronmult_hip.h"
 
@@ -361,6 +362,111 @@ char const *kernel_name(int d, int n, int nb)
     }
 }
 
+// ---- adjoint ------------------------------------------------------------------------------------------
+// Largest dynamic LDS one workgroup may take on the current device (gfx950: 160 KiB).
+size_t device_max_lds()
+{
+    static std::atomic<long> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 64 * 1024;
+    long v = cache[dev].load(std::memory_order_relaxed);
+    if (v == 0)
+    {
+        int a = 0;
+        if (hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || a <= 0)
+            a = 64 * 1024;
+        v = a;
+        cache[dev].store(v, std::memory_order_relaxed);
+    }
+    return static_cast<size_t>(v);
+}
+
+struct AdjointPlan
+{
+    bool in_lds = false, store = false;
+    int threads = 64;
+    size_t lds = 0;
+    long ws_elems = 0;  // per entry, 0 when the workspace is not read
+};
+
+// intermediates: KRONMULT_ADJOINT_AUTO / _RECOMPUTE / _STORE
+hipError_t adjoint_plan(Shape const &s, int elem, int intermediates, size_t max_lds, AdjointPlan &p)
+{
+    if (s.n > kronadj::kMaxN) return hipErrorInvalidValue;
+    if (intermediates < KRONMULT_ADJOINT_AUTO || intermediates > KRONMULT_ADJOINT_STORE) return hipErrorInvalidValue;
+    long const nfib = s.size / s.n;
+    p.threads = static_cast<int>(((nfib + 63) / 64) * 64);
+    if (p.threads > kronadj::kThreads) p.threads = kronadj::kThreads;
+    size_t const red = static_cast<size_t>(p.threads / 64) * 256 * elem;  // per-wavefront 16 x 16 tiles
+    auto fits = [&](long vectors) { return static_cast<size_t>(vectors * s.size * elem) + red <= max_lds; };
+    long const store_vectors = s.d;  // H and X_0..X_{d-2}
+    bool store = intermediates == KRONMULT_ADJOINT_STORE;
+    if (intermediates == KRONMULT_ADJOINT_AUTO) store = fits(store_vectors);
+    long const vectors = store ? store_vectors : 2;
+    p.store = store;
+    p.in_lds = fits(vectors);
+    p.lds = (p.in_lds ? static_cast<size_t>(vectors * s.size * elem) : 0) + red;
+    p.ws_elems = p.in_lds ? 0 : vectors * s.size;
+    return hipSuccess;
+}
+
+template<typename T, int N, bool InLds, bool Store>
+hipError_t adjoint_launch_n(Shape const &s, AdjointPlan const &p, T const *const *mats, T const *const *in,
+                            T const *const *gout, T *const *gin, T *const *gmats, T *const *ws, hipStream_t stream)
+{
+    hipLaunchKernelGGL((kronadj::kron_adjoint_kernel<T, N, InLds, Store>), dim3(s.nb), dim3(p.threads), p.lds,
+                       stream, s.d, static_cast<int>(s.size), mats, s.stride, in, gout, gin, gmats, ws);
+    return hipGetLastError();
+}
+
+template<typename T, int N>
+hipError_t adjoint_launch_mode(Shape const &s, AdjointPlan const &p, T const *const *mats, T const *const *in,
+                               T const *const *gout, T *const *gin, T *const *gmats, T *const *ws, hipStream_t stream)
+{
+    if (p.in_lds)
+        return p.store ? adjoint_launch_n<T, N, true, true>(s, p, mats, in, gout, gin, gmats, ws, stream)
+                       : adjoint_launch_n<T, N, true, false>(s, p, mats, in, gout, gin, gmats, ws, stream);
+    return p.store ? adjoint_launch_n<T, N, false, true>(s, p, mats, in, gout, gin, gmats, ws, stream)
+                   : adjoint_launch_n<T, N, false, false>(s, p, mats, in, gout, gin, gmats, ws, stream);
+}
+
+template<typename T>
+hipError_t kronmult_adjoint_launch(int d, int n, T const *const *mats, int stride, T const *const *in,
+                                   T const *const *gout, T *const *gin, T *const *gmats, T *const *ws, int nb,
+                                   int intermediates, hipStream_t stream)
+{
+    Shape s{};
+    hipError_t err = validate(d, n, stride, nb, s);
+    if (err != hipSuccess) return err;
+    AdjointPlan p;
+    err = adjoint_plan(s, sizeof(T), intermediates, device_max_lds(), p);
+    if (err != hipSuccess) return err;
+    if (nb == 0 || (gin == nullptr && gmats == nullptr)) return hipSuccess;
+    if (mats == nullptr || gout == nullptr || (gmats != nullptr && in == nullptr)) return hipErrorInvalidValue;
+    if (p.ws_elems > 0 && ws == nullptr) return hipErrorInvalidValue;
+    switch (n)
+    {
+    case 1: return adjoint_launch_mode<T, 1>(s, p, mats, in, gout, gin, gmats, ws, stream);
+    case 2: return adjoint_launch_mode<T, 2>(s, p, mats, in, gout, gin, gmats, ws, stream);
+    case 3: return adjoint_launch_mode<T, 3>(s, p, mats, in, gout, gin, gmats, ws, stream);
+    case 4: return adjoint_launch_mode<T, 4>(s, p, mats, in, gout, gin, gmats, ws, stream);
+    case 5: return adjoint_launch_mode<T, 5>(s, p, mats, in, gout, gin, gmats, ws, stream);
+    case 6: return adjoint_launch_mode<T, 6>(s, p, mats, in, gout, gin, gmats, ws, stream);
+    case 7: return adjoint_launch_mode<T, 7>(s, p, mats, in, gout, gin, gmats, ws, stream);
+    case 8: return adjoint_launch_mode<T, 8>(s, p, mats, in, gout, gin, gmats, ws, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template<typename T>
+int kronmult_adjoint_sync(int d, int n, T const *const *mats, int stride, T const *const *in, T const *const *gout,
+                          T **gin, T **gmats, T **ws, int nb, int intermediates)
+{
+    hipError_t err = kronmult_adjoint_launch<T>(d, n, mats, stride, in, gout, gin, gmats, ws, nb, intermediates, nullptr);
+    if (err != hipSuccess || nb == 0) return static_cast<int>(err);
+    return static_cast<int>(hipDeviceSynchronize());
+}
+
 } // namespace
 
 extern "C"
@@ -440,9 +546,65 @@ int kronmult_hip_launch_info(int matrix_count, int matrix_size, int element_size
     return 0;
 }
 
+int kronmult_batched_adjoint_f64(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                                 int matrix_stride, double const *const *input_batched,
+                                 double const *const *grad_output_batched, double **grad_input_batched,
+                                 double **grad_matrix_batched, double **workspace_batched, int nb_batch,
+                                 int intermediates)
+{
+    return kronmult_adjoint_sync<double>(matrix_count, matrix_size, matrix_list_batched, matrix_stride, input_batched,
+                                         grad_output_batched, grad_input_batched, grad_matrix_batched,
+                                         workspace_batched, nb_batch, intermediates);
+}
+
+int kronmult_batched_adjoint_f32(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
+                                 int matrix_stride, float const *const *input_batched,
+                                 float const *const *grad_output_batched, float **grad_input_batched,
+                                 float **grad_matrix_batched, float **workspace_batched, int nb_batch,
+                                 int intermediates)
+{
+    return kronmult_adjoint_sync<float>(matrix_count, matrix_size, matrix_list_batched, matrix_stride, input_batched,
+                                        grad_output_batched, grad_input_batched, grad_matrix_batched,
+                                        workspace_batched, nb_batch, intermediates);
+}
+
+int kronmult_batched_adjoint_f64_async(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                                       int matrix_stride, double const *const *input_batched,
+                                       double const *const *grad_output_batched, double **grad_input_batched,
+                                       double **grad_matrix_batched, double **workspace_batched, int nb_batch,
+                                       int intermediates, void *stream)
+{
+    return static_cast<int>(kronmult_adjoint_launch<double>(
+        matrix_count, matrix_size, matrix_list_batched, matrix_stride, input_batched, grad_output_batched,
+        grad_input_batched, grad_matrix_batched, workspace_batched, nb_batch, intermediates,
+        static_cast<hipStream_t>(stream)));
+}
+
+int kronmult_batched_adjoint_f32_async(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
+                                       int matrix_stride, float const *const *input_batched,
+                                       float const *const *grad_output_batched, float **grad_input_batched,
+                                       float **grad_matrix_batched, float **workspace_batched, int nb_batch,
+                                       int intermediates, void *stream)
+{
+    return static_cast<int>(kronmult_adjoint_launch<float>(
+        matrix_count, matrix_size, matrix_list_batched, matrix_stride, input_batched, grad_output_batched,
+        grad_input_batched, grad_matrix_batched, workspace_batched, nb_batch, intermediates,
+        static_cast<hipStream_t>(stream)));
+}
+
+long long kronmult_batched_adjoint_workspace(int matrix_count, int matrix_size, int element_size, int intermediates)
+{
+    Shape s{};
+    if (element_size != 4 && element_size != 8) return -1;
+    if (validate(matrix_count, matrix_size, matrix_size, 0, s) != hipSuccess) return -1;
+    AdjointPlan p;
+    if (adjoint_plan(s, element_size, intermediates, device_max_lds(), p) != hipSuccess) return -1;
+    return p.ws_elems;
+}
+
 char const *kronmult_hip_version(void)
 {
-    return "kronmult_hip 0.2 (gfx950)";
+    return "kronmult_hip 0.3 (gfx950)";
 }
 
 char const *kronmult_hip_kernel_name(int matrix_count, int matrix_size, int element_size, int nb_batch)

@@ -73,6 +73,47 @@ class DeviceBatch:
         else:
             api.kronmult_batched_async(*args, dtype=self.dtype_name, stream=stream)
 
+    # ---- adjoint -----------------------------------------------------------------------------------------
+    def adjoint_setup(self, grad_out: Optional[np.ndarray] = None, want_input: bool = True,
+                      want_mats: bool = True, intermediates="auto") -> None:
+        """Device buffers for the adjoint: the cotangent pool (one vector per output, default the batch's
+        own), gradient pools shaped like the input and factor pools, and the workspace it needs."""
+        torch = _torch()
+        hb = self.hb
+        g = hb.grad_out_pool if grad_out is None else grad_out
+        if g is None:
+            raise ValueError("no cotangent for this batch")
+        self.gout = torch.from_numpy(np.ascontiguousarray(g, dtype=hb.dtype)).to(self.device)
+        self.gout_ptrs = self._ptrs(self.gout, hb.out_off)
+        self.gin = torch.zeros_like(self.inp) if want_input else None
+        self.gin_ptrs = self._ptrs(self.gin, hb.in_off) if want_input else None
+        self.gmat = torch.zeros_like(self.mat) if want_mats else None
+        self.gmat_ptrs = self._ptrs(self.gmat, hb.mat_off) if want_mats else None
+        self.intermediates = intermediates
+        ws = api.adjoint_workspace(self.d, self.n, self.itemsize, intermediates)
+        if ws > 0:
+            self.adj_ws = torch.zeros(self.nb * ws, dtype=self.out.dtype, device=self.device)
+            self.adj_ws_ptrs = self._ptrs(self.adj_ws, np.arange(self.nb, dtype=np.int64) * ws)
+        else:
+            self.adj_ws = None
+            self.adj_ws_ptrs = None
+
+    def reset_gradients(self) -> None:
+        for t in (self.gin, self.gmat):
+            if t is not None:
+                t.zero_()
+
+    def run_adjoint(self, sync: bool = True, stream=None) -> None:
+        api.kronmult_batched_adjoint(self.d, self.n, self.mat_ptrs, self.stride, self.in_ptrs, self.gout_ptrs,
+                                     self.gin_ptrs, self.gmat_ptrs, self.adj_ws_ptrs, self.nb,
+                                     dtype=self.dtype_name, intermediates=self.intermediates, stream=stream,
+                                     sync=sync)
+
+    def gradients(self):
+        gi = None if self.gin is None else self.gin.detach().cpu().numpy()
+        gm = None if self.gmat is None else self.gmat.detach().cpu().numpy()
+        return gi, gm
+
     def output(self) -> np.ndarray:
         return self.out.detach().cpu().numpy()
 
