@@ -39,6 +39,8 @@ WORKLOADS = {
     "c2": ("ASGarD-like 6D operator apply: d=6 n=4 stride=64, 128 output x 64 input elements (nb=8192), "
            "factors = digit-indexed 4x4 blocks of six 64x64 N(0,1)/8 matrices, 64 shared inputs of 4096, "
            "128 outputs x 64 contributions"),
+    "c3": ("mixed (d, n, count) groups (2,8,20000) (3,6,6000) (4,4,3000) (5,3,1500) (6,2,800), contiguous "
+           "factors, 4 consecutive entries per output, one grouped call (strategy and shapes autotuned)"),
     "c4": "d=4 n=8 stride=8, nb=16384, factors U[-1,1]*10^U[-2,2], 1024 outputs x 16 contributions",
     "c5": "d=3 n=8 stride=8, nb=8192, 256 shared factor triples, 2048 outputs x 4 contributions",
 }
@@ -59,32 +61,39 @@ def cpu_threads() -> int:
     return avail
 
 
-def cpu_baseline(hb, runs: int):
-    """The reference's kronmult_omp (built from its header, its own flags) on all host threads, full batch."""
+def cpu_baseline(hbs, runs: int):
+    """The reference's kronmult_omp (built from its header, its own flags) on all host threads, one call per
+    batch (a grouped workload is one call per group, back to back), full batches."""
     import oracle
     threads = cpu_threads()
-    kind = "reference"
-    if oracle.have_ref("shipped"):
-        runner = oracle.RefRunner(hb, "shipped")
-        call = lambda: runner.run(threads)
-        reset = runner.reset
-    else:  # the restatement, same arithmetic and threading scheme
-        kind = "port"
-        out = hb.output_pool_init()
-        call = lambda: oracle.kronmult_omp(hb, threads, out_pool=out)
-        reset = lambda: None
+    kind = "reference" if oracle.have_ref("shipped") else "port"
+    calls, resets = [], []
+    for hb in hbs:
+        if kind == "reference":
+            runner = oracle.RefRunner(hb, "shipped")
+            calls.append(lambda r=runner: r.run(threads))
+            resets.append(runner.reset)
+        else:  # the restatement, same arithmetic and threading scheme
+            out = hb.output_pool_init()
+            calls.append(lambda hb=hb, out=out: oracle.kronmult_omp(hb, threads, out_pool=out))
+            resets.append(lambda: None)
     times = []
     for r in range(runs + 1):
-        reset()
+        for reset in resets:
+            reset()
         t0 = time.perf_counter()
-        call()
+        for call in calls:
+            call()
         t1 = time.perf_counter()
         if r > 0:
             times.append(t1 - t0)
         log(f"[cpu] run {r} {'(warm-up) ' if r == 0 else ''}{(t1 - t0) * 1e3:.1f} ms on {threads} threads")
     med = statistics.median(times)
-    return {"value": round(hb.flops() / med / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": kind,
-            "sample": (f"full {hb.name} batch ({hb.nb} entries, {hb.flops() / 1e9:.3f} GFLOP), private input "
+    flops = sum(hb.flops() for hb in hbs)
+    names = "+".join(hb.name for hb in hbs)
+    nb = sum(hb.nb for hb in hbs)
+    return {"value": round(flops / med / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": kind,
+            "sample": (f"full {names} batch ({nb} entries, {flops / 1e9:.3f} GFLOP), private input "
                        f"copies, 1 warm-up + {runs} runs, median {med * 1e3:.1f} ms, min {min(times) * 1e3:.1f} ms")}
 
 
@@ -129,14 +138,31 @@ def main() -> int:
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
     np_dtype = np.float64 if args.dtype == "f64" else np.float32
-    hb = W.by_name(args.config).astype(np_dtype)
-    log(f"[bench] {hb.name} {args.dtype}: d={hb.d} n={hb.n} nb={hb.nb} kernel="
-        f"{api.kernel_name(hb.d, hb.n, hb.dtype.itemsize, hb.nb)} on {torch.cuda.get_device_name(dev)}")
-    db = DeviceBatch(hb, device=str(dev))
+    batches = W.by_name(args.config)
+    batches = [b.astype(np_dtype) for b in (batches if isinstance(batches, list) else [batches])]
+    dbs = [DeviceBatch(hb, device=str(dev)) for hb in batches]
     stream = torch.cuda.current_stream(dev)
+    extra_config = {}
+    if len(dbs) == 1:
+        hb, db = batches[0], dbs[0]
+        kernel = api.kernel_name(hb.d, hb.n, hb.dtype.itemsize, hb.nb)
+        step = lambda s: db.run(sync=False, stream=s)
+        extra_config = {"d": hb.d, "n": hb.n, "matrix_stride": hb.stride, "nb_batch": hb.nb,
+                        "distinct_outputs": hb.out_count}
+    else:
+        from kronmult993_amd.autotune import Autotuner, grouped_call_from_choice
+        choice = Autotuner().tune_grouped(dbs)
+        for db in dbs:
+            db.reset_output()
+        groups, structs, kw = grouped_call_from_choice(dbs, choice)
+        kernel = "group_persistent" if choice["strategy"] == "persistent" else "separate launches"
+        step = lambda s: api.kronmult_batched_grouped(groups, dtype=args.dtype, stream=s, structs=structs, **kw)
+        extra_config = {"groups": [[hb.d, hb.n, hb.nb] for hb in batches], "autotuned": choice}
+    log(f"[bench] {'+'.join(hb.name for hb in batches)} {args.dtype}: kernel={kernel} on "
+        f"{torch.cuda.get_device_name(dev)}")
 
     for _ in range(args.warmup):
-        db.run(sync=False, stream=stream)
+        step(stream)
     torch.cuda.synchronize(dev)
 
     # eager pass: HIP events around every launch on the launch stream -> kernel duration
@@ -144,7 +170,7 @@ def main() -> int:
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     for i in range(args.steps):
         starts[i].record(stream)
-        db.run(sync=False, stream=stream)
+        step(stream)
         ends[i].record(stream)
     torch.cuda.synchronize(dev)
     kernel_ms = [a.elapsed_time(b) for a, b in zip(starts, ends)]
@@ -157,7 +183,7 @@ def main() -> int:
     with torch.cuda.stream(cap):
         with torch.cuda.graph(graph, stream=cap):
             for _ in range(args.steps):
-                db.run(sync=False, stream=cap)
+                step(cap)
     torch.cuda.synchronize(dev)
     graph.replay()  # warm replay
     torch.cuda.synchronize(dev)
@@ -174,13 +200,12 @@ def main() -> int:
     log(f"[bench] {args.steps} steps: {ms_per_step * 1e3:.2f} us/step (graph replay), kernel avg "
         f"{k_avg * 1e3:.2f} us (eager, min {min(kernel_ms) * 1e3:.2f}), wall {wall * 1e6 / args.steps:.2f} us/step")
 
-    flops = hb.flops()
-    contract = hb.contract_bytes()
+    flops = sum(hb.flops() for hb in batches)
+    contract = sum(hb.contract_bytes() for hb in batches)
     gflops = flops / (ms_per_step * 1e-3) / 1e9
     achieved_gbs = contract / (k_avg * 1e-3) / 1e9
     peak_tf = FP64_PEAK_TFLOPS if args.dtype == "f64" else FP32_PEAK_TFLOPS
     achieved_tf = flops / (k_avg * 1e-3) / 1e12
-    kernel = api.kernel_name(hb.d, hb.n, hb.dtype.itemsize, hb.nb)
     result = {
         "metric": "batched kron-matvec GFLOP/s & %roofline, d=6 n=4 FP64 vs FP32-split at 1e-10",
         "value": round(gflops, 2),
@@ -194,9 +219,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": args.dtype,
         "data": "synthetic (seeded counter-based generator, seed 0)",
-        "config": {"workload": f"{hb.name}: {WORKLOADS[args.config]}", "d": hb.d, "n": hb.n,
-                   "matrix_stride": hb.stride, "nb_batch": hb.nb, "distinct_outputs": hb.out_count,
-                   "kernel": kernel},
+        "config": dict({"workload": f"{args.config}: {WORKLOADS[args.config]}", "kernel": kernel},
+                       **extra_config),
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic(f"{args.config}_{args.dtype}", kernel)},
@@ -208,7 +232,7 @@ def main() -> int:
         "flops_per_launch": flops,
     }
     if args.cpu_baseline == "auto":
-        result["cpu_baseline"] = cpu_baseline(hb, args.cpu_runs)
+        result["cpu_baseline"] = cpu_baseline(batches, args.cpu_runs)
     if args.calibrate_copy:
         src = torch.ones(256 * 1024 * 1024 // 8, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)

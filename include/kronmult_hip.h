@@ -144,6 +144,59 @@ int kronmult_batched_adjoint_f32_async(int matrix_count, int matrix_size, float 
 long long kronmult_batched_adjoint_workspace(int matrix_count, int matrix_size, int element_size, int intermediates);
 
 /*
+ * Several batches of different (d, n) in one call (no counterpart in the reference, which takes one
+ * (d, n, stride) per call: kronmult_gpu/kronmult.cu:173-197). Each group has the arguments of one
+ * kronmult_batched call plus launch hints; outputs may alias within and across groups. Stream-ordered on
+ * `stream` (NULL = null stream), no synchronisation, no allocation, nothing written but the outputs: the
+ * call can be captured in a hipGraph and replayed.
+ *
+ * Strategies (options->strategy; options may be NULL = AUTO with defaults):
+ *   KRONMULT_GROUPED_PERSISTENT  one persistent launch over every group with n <= 8 whose vector and
+ *                                factors fit in 64 KiB of LDS; `workgroups_per_cu` sizes its grid (0 =
+ *                                default 2) and each group's `entries_per_tile` its tiles (0 = one pass);
+ *                                any other group gets a launch of its own, as below;
+ *   KRONMULT_GROUPED_SEPARATE    one ordinary launch per group, shaped by that group's `tuning`;
+ *   KRONMULT_GROUPED_AUTO        currently the persistent strategy.
+ * Only the first 16 groups can take part in the persistent launch; later groups get launches of their own.
+ */
+#define KRONMULT_GROUPED_AUTO 0
+#define KRONMULT_GROUPED_PERSISTENT 1
+#define KRONMULT_GROUPED_SEPARATE 2
+
+typedef struct kronmult_grouped_options
+{
+    int strategy;
+    int workgroups_per_cu;
+} kronmult_grouped_options_t;
+
+typedef struct kronmult_group_f64
+{
+    int matrix_count, matrix_size, matrix_stride, nb_batch;
+    double const *const *matrix_list_batched;
+    double **input_batched;
+    double **output_batched;
+    double **workspace_batched;
+    kronmult_tuning_t tuning; /* launch shape when the group runs as a launch of its own */
+    int entries_per_tile;     /* persistent launch: consecutive entries per tile (0 = default) */
+} kronmult_group_f64_t;
+
+typedef struct kronmult_group_f32
+{
+    int matrix_count, matrix_size, matrix_stride, nb_batch;
+    float const *const *matrix_list_batched;
+    float **input_batched;
+    float **output_batched;
+    float **workspace_batched;
+    kronmult_tuning_t tuning;
+    int entries_per_tile;
+} kronmult_group_f32_t;
+
+int kronmult_batched_grouped_f64(kronmult_group_f64_t const *groups, int group_count,
+                                 kronmult_grouped_options_t const *options, void *stream);
+int kronmult_batched_grouped_f32(kronmult_group_f32_t const *groups, int group_count,
+                                 kronmult_grouped_options_t const *options, void *stream);
+
+/*
  * Library version string and the name of the kernel the dispatcher would pick for (d, n, element size in
  * bytes, nb_batch); both point to static storage. No reference counterpart (diagnostics only).
  */

@@ -34,6 +34,21 @@ class LaunchInfo(ctypes.Structure):
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
+class GroupedOptions(ctypes.Structure):
+    """kronmult_grouped_options_t: strategy (0 auto, 1 persistent, 2 separate), workgroups per CU."""
+    _fields_ = [("strategy", _I), ("workgroups_per_cu", _I)]
+
+
+def _group_struct(name):
+    return type(name, (ctypes.Structure,), {"_fields_": [
+        ("matrix_count", _I), ("matrix_size", _I), ("matrix_stride", _I), ("nb_batch", _I),
+        ("matrix_list_batched", _P), ("input_batched", _P), ("output_batched", _P), ("workspace_batched", _P),
+        ("tuning", Tuning), ("entries_per_tile", _I)]})
+
+
+GroupF64 = _group_struct("GroupF64")  # kronmult_group_f64_t
+GroupF32 = _group_struct("GroupF32")  # kronmult_group_f32_t
+
 # name -> (restype, argtypes); the order of arguments is the one of include/kronmult_hip.h
 SIGNATURES = {
     "kronmult_batched_f64": (_I, [_I, _I, _P, _I, _P, _P, _P, _I]),
@@ -48,6 +63,8 @@ SIGNATURES = {
     "kronmult_batched_adjoint_f64_async": (_I, [_I, _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _P]),
     "kronmult_batched_adjoint_f32_async": (_I, [_I, _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _P]),
     "kronmult_batched_adjoint_workspace": (ctypes.c_longlong, [_I, _I, _I, _I]),
+    "kronmult_batched_grouped_f64": (_I, [ctypes.POINTER(GroupF64), _I, ctypes.POINTER(GroupedOptions), _P]),
+    "kronmult_batched_grouped_f32": (_I, [ctypes.POINTER(GroupF32), _I, ctypes.POINTER(GroupedOptions), _P]),
     "kronmult_hip_version": (ctypes.c_char_p, []),
     "kronmult_hip_kernel_name": (ctypes.c_char_p, [_I, _I, _I, _I]),
 }

@@ -16,7 +16,7 @@ from typing import Any, Optional
 
 import ctypes
 
-from ._lib import LaunchInfo, Tuning, lib
+from ._lib import GroupedOptions, GroupF32, GroupF64, LaunchInfo, Tuning, lib
 
 HIP_SUCCESS = 0
 HIP_ERROR_INVALID_VALUE = 1
@@ -152,6 +152,45 @@ def adjoint_workspace(matrix_count: int, matrix_size: int, element_size: int, in
     if v < 0:
         raise KronmultError(HIP_ERROR_INVALID_VALUE, "kronmult_batched_adjoint_workspace")
     return int(v)
+
+
+GROUPED_AUTO, GROUPED_PERSISTENT, GROUPED_SEPARATE = 0, 1, 2
+_STRATEGIES = {"auto": GROUPED_AUTO, "persistent": GROUPED_PERSISTENT, "separate": GROUPED_SEPARATE}
+
+
+class Group:
+    """Arguments of one kronmult_batched call inside a grouped call, plus launch hints (`tuning` for a
+    launch of its own, `entries_per_tile` for the persistent launch)."""
+
+    def __init__(self, matrix_count, matrix_size, matrix_list_batched, matrix_stride, input_batched,
+                 output_batched, workspace_batched, nb_batch, tuning=None, entries_per_tile=0):
+        self.args = (matrix_count, matrix_size, matrix_stride, nb_batch, _addr(matrix_list_batched),
+                     _addr(input_batched), _addr(output_batched), _addr(workspace_batched))
+        self.tuning = _tuning(tuning) or Tuning(0, 0, 0)
+        self.entries_per_tile = int(entries_per_tile)
+
+
+def grouped_structs(groups, dtype):
+    """The ctypes array of kronmult_group_*_t for `groups` (kept by the caller while it is in use)."""
+    S = GroupF64 if _suffix(dtype) == "f64" else GroupF32
+    arr = (S * max(1, len(groups)))()
+    for i, g in enumerate(groups):
+        d, n, stride, nb, m, x, y, w = g.args
+        arr[i] = S(d, n, stride, nb, m, x, y, w, g.tuning, g.entries_per_tile)
+    return arr
+
+
+def kronmult_batched_grouped(groups, *, dtype, strategy="auto", workgroups_per_cu: int = 0, stream=None,
+                             structs=None) -> None:
+    """Stream-ordered grouped call (no synchronisation). `structs`: a prebuilt grouped_structs() array
+    (saves rebuilding it on every call)."""
+    arr = structs if structs is not None else grouped_structs(groups, dtype)
+    st = _STRATEGIES[strategy] if isinstance(strategy, str) else int(strategy)
+    opt = GroupedOptions(st, int(workgroups_per_cu))
+    f = getattr(lib(), "kronmult_batched_grouped_" + _suffix(dtype))
+    rc = f(arr, len(groups), ctypes.byref(opt), _stream_handle(stream))
+    if rc != HIP_SUCCESS:
+        raise KronmultError(rc, "kronmult_batched_grouped")
 
 
 def kernel_name(matrix_count: int, matrix_size: int, element_size: int, nb_batch: int) -> str:

@@ -24,6 +24,7 @@
 #include <cstdio>
 
 #include "kron_adjoint.hpp"
+#include "kron_group.hpp"
 #include "kron_reg.hpp"
 #include "kronmult_hip.h"
 
@@ -467,6 +468,110 @@ int kronmult_adjoint_sync(int d, int n, T const *const *mats, int stride, T cons
     return static_cast<int>(hipDeviceSynchronize());
 }
 
+// ---- grouped launch -------------------------------------------------------------------------------------
+constexpr size_t kGroupLdsBudget = 64 * 1024;
+
+template<typename T, typename GroupT>
+bool group_persistent_ok(GroupT const &g, Shape const &s)
+{
+    long const per_entry = (s.size + static_cast<long>(s.d) * s.n * s.n) * static_cast<long>(sizeof(T));
+    return s.n <= 8 && per_entry <= static_cast<long>(kGroupLdsBudget);
+}
+
+template<typename T, typename GroupT>
+hipError_t kronmult_grouped_launch(GroupT const *groups, int count, kronmult_grouped_options_t const *options,
+                                   hipStream_t stream)
+{
+    if (count < 0 || (count > 0 && groups == nullptr)) return hipErrorInvalidValue;
+    kronmult_grouped_options_t const opt = options ? *options : kronmult_grouped_options_t{0, 0};
+    if (opt.strategy < KRONMULT_GROUPED_AUTO || opt.strategy > KRONMULT_GROUPED_SEPARATE || opt.workgroups_per_cu < 0)
+        return hipErrorInvalidValue;
+    // validate everything before launching anything
+    Shape shapes[krongroup::kMaxGroups];
+    bool persistent[krongroup::kMaxGroups];
+    int n_persistent = 0;
+    for (int i = 0; i < count; ++i)
+    {
+        GroupT const &g = groups[i];
+        Shape s{};
+        hipError_t err = validate(g.matrix_count, g.matrix_size, g.matrix_stride, g.nb_batch, s);
+        if (err != hipSuccess) return err;
+        if (g.entries_per_tile < 0) return hipErrorInvalidValue;
+        if (g.nb_batch > 0 && (g.matrix_list_batched == nullptr || g.input_batched == nullptr ||
+                               g.output_batched == nullptr))
+            return hipErrorInvalidValue;
+        // groups past the first kMaxGroups always get launches of their own
+        bool const pers = opt.strategy != KRONMULT_GROUPED_SEPARATE && i < krongroup::kMaxGroups &&
+                          group_persistent_ok<T>(g, s) && g.nb_batch > 0;
+        if (i < krongroup::kMaxGroups)
+        {
+            shapes[i] = s;
+            persistent[i] = pers;
+        }
+        if (!pers)
+        {
+            Launch L;
+            kronmult_tuning_t const tu = g.tuning;
+            err = kronmult_launch<T>(g.matrix_count, g.matrix_size, g.matrix_list_batched, g.matrix_stride,
+                                     g.input_batched, g.output_batched, g.workspace_batched, 0, &tu, nullptr, &L);
+            if (err != hipSuccess) return err;
+            if (L.workspace && g.nb_batch > 0 && g.workspace_batched == nullptr) return hipErrorInvalidValue;
+        }
+        n_persistent += pers ? 1 : 0;
+    }
+    // separate launches first (same stream, so the order does not matter for the sums)
+    for (int i = 0; i < count; ++i)
+    {
+        GroupT const &g = groups[i];
+        if (i < krongroup::kMaxGroups && persistent[i]) continue;
+        if (g.nb_batch == 0) continue;
+        hipError_t err = kronmult_launch<T>(g.matrix_count, g.matrix_size, g.matrix_list_batched, g.matrix_stride,
+                                            g.input_batched, g.output_batched, g.workspace_batched, g.nb_batch,
+                                            &g.tuning, stream, nullptr);
+        if (err != hipSuccess) return err;
+    }
+    if (n_persistent == 0) return hipSuccess;
+
+    krongroup::GroupTable<T> table{};
+    size_t lds = 0;
+    int tiles = 0;
+    for (int i = 0; i < count && i < krongroup::kMaxGroups; ++i)
+    {
+        if (!persistent[i]) continue;
+        GroupT const &g = groups[i];
+        Shape const &s = shapes[i];
+        krongroup::GroupDesc<T> &D = table.g[table.count];
+        D.d = s.d;
+        D.n = s.n;
+        D.stride = s.stride;
+        D.nb = s.nb;
+        D.size = static_cast<int>(s.size);
+        D.nfib = static_cast<int>(s.size / s.n);
+        long const per_entry = (s.size + static_cast<long>(s.d) * s.n * s.n) * static_cast<long>(sizeof(T));
+        int E = krongroup::kThreads / D.nfib;
+        if (E < 1) E = 1;
+        while (E > 1 && E * per_entry > static_cast<long>(kGroupLdsBudget)) --E;
+        D.E = E;
+        D.epw = g.entries_per_tile > 0 ? g.entries_per_tile : E;
+        D.mats = g.matrix_list_batched;
+        D.in = const_cast<T const *const *>(g.input_batched);
+        D.out = g.output_batched;
+        table.tile_start[table.count] = tiles;
+        tiles += (s.nb + D.epw - 1) / D.epw;
+        size_t const need = static_cast<size_t>(E) * per_entry;
+        if (need > lds) lds = need;
+        ++table.count;
+    }
+    table.tile_start[table.count] = tiles;
+    table.tiles = tiles;
+    int const per_cu = opt.workgroups_per_cu > 0 ? opt.workgroups_per_cu : 2;
+    long grid = static_cast<long>(device_cu_count()) * per_cu;
+    if (grid > tiles) grid = tiles;
+    hipLaunchKernelGGL(krongroup::kron_group_kernel<T>, dim3(static_cast<unsigned>(grid)), dim3(krongroup::kThreads),
+                       lds, stream, table);
+    return hipGetLastError();
+}
+
 } // namespace
 
 extern "C"
@@ -600,6 +705,20 @@ long long kronmult_batched_adjoint_workspace(int matrix_count, int matrix_size, 
     AdjointPlan p;
     if (adjoint_plan(s, element_size, intermediates, device_max_lds(), p) != hipSuccess) return -1;
     return p.ws_elems;
+}
+
+int kronmult_batched_grouped_f64(kronmult_group_f64_t const *groups, int group_count,
+                                 kronmult_grouped_options_t const *options, void *stream)
+{
+    return static_cast<int>(
+        kronmult_grouped_launch<double>(groups, group_count, options, static_cast<hipStream_t>(stream)));
+}
+
+int kronmult_batched_grouped_f32(kronmult_group_f32_t const *groups, int group_count,
+                                 kronmult_grouped_options_t const *options, void *stream)
+{
+    return static_cast<int>(
+        kronmult_grouped_launch<float>(groups, group_count, options, static_cast<hipStream_t>(stream)));
 }
 
 char const *kronmult_hip_version(void)

@@ -114,6 +114,11 @@ class DeviceBatch:
         gm = None if self.gmat is None else self.gmat.detach().cpu().numpy()
         return gi, gm
 
+    def group(self, tuning=None, entries_per_tile: int = 0) -> "api.Group":
+        """This batch as one group of a grouped call."""
+        return api.Group(self.d, self.n, self.mat_ptrs, self.stride, self.in_ptrs, self.out_ptrs, self.ws_ptrs,
+                         self.nb, tuning=tuning, entries_per_tile=entries_per_tile)
+
     def output(self) -> np.ndarray:
         return self.out.detach().cpu().numpy()
 
