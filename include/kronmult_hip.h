@@ -197,6 +197,28 @@ int kronmult_batched_grouped_f32(kronmult_group_f32_t const *groups, int group_c
                                  kronmult_grouped_options_t const *options, void *stream);
 
 /*
+ * FP64 data, contractions on the FP32 matrix cores (no counterpart in the reference, which has plain FP64
+ * and FP32 only: kronmult_gpu/kronmult.cu:202-224). Same data contract as kronmult_batched_f64 (FP64
+ * factors, inputs and outputs, outputs accumulated in FP64), stream-ordered on `stream`; n <= 8 and
+ * n^d <= 7936. split_mode picks how FP64 operands are mapped onto FP32 matrix-core products:
+ *   KRONMULT_SPLIT_FP32      operands rounded to FP32, FP32 accumulation            (~1e-7 relative)
+ *   KRONMULT_SPLIT_PAIR32    FP32 hi + lo pairs, three products, FP32 accumulation  (~1e-7: accumulation-bound)
+ *   KRONMULT_SPLIT_BF16      operands rounded to BF16, FP32 accumulation            (~1e-3)
+ *   KRONMULT_SPLIT_PAIRBF16  BF16 hi + lo pairs, three products, FP32 accumulation  (~1e-5)
+ *   KRONMULT_SPLIT_FIXED3    error-free fixed-point slicing (hi, mid, lo per fiber / per factor row),
+ *                            exact matrix-core sums, FP64 recombination             (~1e-13)
+ */
+#define KRONMULT_SPLIT_FP32 0
+#define KRONMULT_SPLIT_PAIR32 1
+#define KRONMULT_SPLIT_BF16 2
+#define KRONMULT_SPLIT_PAIRBF16 3
+#define KRONMULT_SPLIT_FIXED3 4
+
+int kronmult_batched_split_f64(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                               int matrix_stride, double **input_batched, double **output_batched, int nb_batch,
+                               int split_mode, void *stream);
+
+/*
  * Library version string and the name of the kernel the dispatcher would pick for (d, n, element size in
  * bytes, nb_batch); both point to static storage. No reference counterpart (diagnostics only).
  */

@@ -193,6 +193,21 @@ def kronmult_batched_grouped(groups, *, dtype, strategy="auto", workgroups_per_c
         raise KronmultError(rc, "kronmult_batched_grouped")
 
 
+SPLIT_MODES = {"fp32": 0, "pair32": 1, "bf16": 2, "pairbf16": 3, "fixed3": 4}
+
+
+def kronmult_batched_split(matrix_count: int, matrix_size: int, matrix_list_batched, matrix_stride: int,
+                           input_batched, output_batched, nb_batch: int, *, mode="fixed3", stream=None) -> None:
+    """FP64 data, contractions on the FP32 matrix cores (`mode`: fp32, pair32, bf16, pairbf16, fixed3);
+    stream-ordered."""
+    m = SPLIT_MODES[mode] if isinstance(mode, str) else int(mode)
+    rc = lib().kronmult_batched_split_f64(matrix_count, matrix_size, _addr(matrix_list_batched), matrix_stride,
+                                          _addr(input_batched), _addr(output_batched), nb_batch, m,
+                                          _stream_handle(stream))
+    if rc != HIP_SUCCESS:
+        raise KronmultError(rc, "kronmult_batched_split_f64")
+
+
 def kernel_name(matrix_count: int, matrix_size: int, element_size: int, nb_batch: int) -> str:
     return lib().kronmult_hip_kernel_name(matrix_count, matrix_size, element_size, nb_batch).decode()
 

@@ -25,6 +25,7 @@
 
 #include "kron_adjoint.hpp"
 #include "kron_group.hpp"
+#include "kron_split.hpp"
 #include "kron_reg.hpp"
 #include "kronmult_hip.h"
 
@@ -572,6 +573,59 @@ hipError_t kronmult_grouped_launch(GroupT const *groups, int count, kronmult_gro
     return hipGetLastError();
 }
 
+// ---- FP64 through FP32 matrix cores ---------------------------------------------------------------------
+template<int MODE, int N>
+hipError_t split_launch(Shape const &s, double const *const *mats, double const *const *in, double *const *out,
+                        hipStream_t stream)
+{
+    long const ntiles = (s.size / s.n + 15) / 16;
+    int const waves = ntiles >= 4 ? 4 : static_cast<int>(ntiles);
+    size_t const lds = static_cast<size_t>(s.size) * sizeof(double) +
+                       static_cast<size_t>(kronsplit::SplitTraits<MODE>::kSlices) * 16 * N * sizeof(float);
+    hipLaunchKernelGGL((kronsplit::kron_split_kernel<MODE, N>), dim3(s.nb), dim3(64 * waves), lds, stream, s.d,
+                       static_cast<int>(s.size), mats, s.stride, in, out);
+    return hipGetLastError();
+}
+
+template<int MODE>
+hipError_t split_dispatch_n(Shape const &s, double const *const *mats, double const *const *in,
+                            double *const *out, hipStream_t stream)
+{
+    switch (s.n)
+    {
+    case 1: return split_launch<MODE, 1>(s, mats, in, out, stream);
+    case 2: return split_launch<MODE, 2>(s, mats, in, out, stream);
+    case 3: return split_launch<MODE, 3>(s, mats, in, out, stream);
+    case 4: return split_launch<MODE, 4>(s, mats, in, out, stream);
+    case 5: return split_launch<MODE, 5>(s, mats, in, out, stream);
+    case 6: return split_launch<MODE, 6>(s, mats, in, out, stream);
+    case 7: return split_launch<MODE, 7>(s, mats, in, out, stream);
+    case 8: return split_launch<MODE, 8>(s, mats, in, out, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t kronmult_split_launch(int d, int n, double const *const *mats, int stride, double *const *in,
+                                 double *const *out, int nb, int mode, hipStream_t stream)
+{
+    Shape s{};
+    hipError_t err = validate(d, n, stride, nb, s);
+    if (err != hipSuccess) return err;
+    if (n > kronsplit::kMaxN || mode < KRONMULT_SPLIT_FP32 || mode > KRONMULT_SPLIT_FIXED3) return hipErrorInvalidValue;
+    if (s.size * static_cast<long>(sizeof(double)) + 3 * 16 * 8 * 4 > 64 * 1024) return hipErrorInvalidValue;
+    if (nb == 0) return hipSuccess;
+    if (mats == nullptr || in == nullptr || out == nullptr) return hipErrorInvalidValue;
+    double const *const *cin = const_cast<double const *const *>(in);
+    switch (mode)
+    {
+    case KRONMULT_SPLIT_FP32: return split_dispatch_n<kronsplit::FP32>(s, mats, cin, out, stream);
+    case KRONMULT_SPLIT_PAIR32: return split_dispatch_n<kronsplit::PAIR32>(s, mats, cin, out, stream);
+    case KRONMULT_SPLIT_BF16: return split_dispatch_n<kronsplit::BF16>(s, mats, cin, out, stream);
+    case KRONMULT_SPLIT_PAIRBF16: return split_dispatch_n<kronsplit::PAIRBF16>(s, mats, cin, out, stream);
+    default: return split_dispatch_n<kronsplit::FIXED3>(s, mats, cin, out, stream);
+    }
+}
+
 } // namespace
 
 extern "C"
@@ -719,6 +773,15 @@ int kronmult_batched_grouped_f32(kronmult_group_f32_t const *groups, int group_c
 {
     return static_cast<int>(
         kronmult_grouped_launch<float>(groups, group_count, options, static_cast<hipStream_t>(stream)));
+}
+
+int kronmult_batched_split_f64(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                               int matrix_stride, double **input_batched, double **output_batched, int nb_batch,
+                               int split_mode, void *stream)
+{
+    return static_cast<int>(kronmult_split_launch(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
+                                                  input_batched, output_batched, nb_batch, split_mode,
+                                                  static_cast<hipStream_t>(stream)));
 }
 
 char const *kronmult_hip_version(void)

@@ -114,6 +114,15 @@ class DeviceBatch:
         gm = None if self.gmat is None else self.gmat.detach().cpu().numpy()
         return gi, gm
 
+    def run_split(self, mode: str = "fixed3", sync: bool = True, stream=None) -> None:
+        """FP64 batch through the FP32 matrix cores (kronmult_batched_split_f64)."""
+        if self.itemsize != 8:
+            raise TypeError("the split path takes FP64 batches")
+        api.kronmult_batched_split(self.d, self.n, self.mat_ptrs, self.stride, self.in_ptrs, self.out_ptrs,
+                                   self.nb, mode=mode, stream=stream)
+        if sync:
+            _torch().cuda.synchronize(self.device)
+
     def group(self, tuning=None, entries_per_tile: int = 0) -> "api.Group":
         """This batch as one group of a grouped call."""
         return api.Group(self.d, self.n, self.mat_ptrs, self.stride, self.in_ptrs, self.out_ptrs, self.ws_ptrs,
