@@ -1,0 +1,337 @@
+// kron_reg_adjoint.hpp -- adjoint on the register-resident layout of kron_plan.hpp, for plans with one mode
+// per phase (RHO = 1: every thread holds one whole fiber of the phase's mode).
+//
+// Same mathematics as kron_adjoint.hpp: d x = H_d with H_0 = g, H_{t+1} = A_t^T on mode t of H_t, and
+// dA_t = sum over the fibers of mode t of h_f x_f^T with x_f from X_t = (A_{t+1} .. A_{d-1} on their modes) x.
+// Here a workgroup of P threads walks a run of entries (inputs and cotangents prefetched one entry ahead,
+// straight into registers in the layouts they are first needed in), and both chains run in registers:
+//   * H chain: g is loaded in the layout of phase d-1 (mode 0 in registers; element j*P + t, coalesced);
+//     step t contracts mode t with A_t^T in registers, then the vector goes through LDS into the layout of
+//     the next faster mode (phase d-2-t);
+//   * X chain: x is loaded in the layout of phase 0 (mode d-1 in registers; elements t*R .. t*R+R-1) and
+//     contracted forward (A_{d-1}, then A_{d-2}, ...) phase by phase. RECOMPUTE rebuilds X_t from the x kept
+//     in registers for every t; STORE runs the chain once and keeps X_0 .. X_{d-2} in LDS;
+//   * dA_t: when mode t is in registers, H_t and X_t are written to LDS fiber-major (row r = index of mode
+//     t, column = fiber, rows padded by one 16-byte granule so different rows sit in different banks); then
+//     thread (r, c, subset s) sums h_f[r] * x_f[c] over its subset of fibers (16-byte loads, broadcast
+//     across the threads that share r or c), subsets are reduced through LDS and one atomic per factor
+//     element is added to the gradient.
+// Gradients of shared storage are summed with hardware atomic adds, as in the forward epilogue.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kron_reg.hpp"
+
+namespace kronregadj
+{
+
+using kronreg::as_constant;
+using kronreg::as_global;
+using kronreg::constant_ptr;
+using kronreg::global_ptr;
+using kronreg::lds_barrier;
+
+template<typename T, typename PL>
+struct AdjShape
+{
+    static_assert(PL::kRho == 1, "one mode per phase");
+    static constexpr int N = PL::kN, D = PL::kD, P = PL::P, R = PL::R, SIZE = PL::SIZE;
+    static constexpr int NN = N * N;
+    static constexpr int NFIB = SIZE / N;  // == P
+    static constexpr int G = 16 / static_cast<int>(sizeof(T));
+    static constexpr int RS = NFIB + G;  // fiber-major row stride
+    static constexpr int FM = N * RS;    // one fiber-major vector
+    static constexpr int NSUB = P / NN;  // fiber subsets of the factor-gradient sums
+    static constexpr int FS = NFIB / NSUB;
+    static constexpr int EX = SIZE > FM ? SIZE : FM;
+    static constexpr bool kCoefLds = kronreg::coef_in_lds<PL>();
+    static constexpr int COEF = kCoefLds ? kronreg::Coef<T, PL>::kCount : 0;
+    static constexpr int RED = NSUB > 1 ? NSUB * NN : 0;
+    static_assert(P % NN == 0 && FS % G == 0, "factor-gradient subsets must tile the fibers");
+    // LDS elements: exchange / X scratch, H scratch, stored X_0..X_{d-2}, staged factors, subset partials
+    static constexpr int lds_elems(bool store) { return EX + FM + (store ? (D - 1) * FM : 0) + COEF + RED; }
+};
+
+// one fiber (R = N values) times A_m or A_m^T; factors staged in LDS as [m][r][k] or read from A directly
+template<typename T, typename PL, bool TRANS>
+__device__ __forceinline__ void contract(T (&v)[PL::R], T const *A_generic, int stride, T const *coef_m)
+{
+    constexpr int N = PL::kN;
+    T x[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) x[k] = v[k];
+    if constexpr (kronreg::coef_in_lds<PL>())
+    {
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+        {
+            T acc = T(0);
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc = fma(x[k], TRANS ? coef_m[k * N + r] : coef_m[r * N + k], acc);
+            v[r] = acc;
+        }
+    }
+    else
+    {
+        constant_ptr<T> A = as_constant(A_generic);
+        T a[N][N];
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+#pragma unroll
+            for (int k = 0; k < N; ++k) a[r][k] = TRANS ? A[k + r * stride] : A[r + k * stride];
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+        {
+            T acc = T(0);
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc = fma(x[k], a[r][k], acc);
+            v[r] = acc;
+        }
+    }
+}
+
+template<typename T, typename PL>
+struct Adj
+{
+    using S = AdjShape<T, PL>;
+    using L = kronreg::Lds<T, PL>;  // for its exchange swizzle
+    static constexpr int D = S::D, R = S::R, P = S::P, N = S::N;
+
+    struct Ctx
+    {
+        T *ex, *ht, *xs, *coef, *red;
+        T const *const *A;  // the entry's factors
+        int stride;
+        T *const *gA;       // the entry's factor-gradient pointers (nullptr: not wanted)
+        int t;              // thread
+    };
+
+    static __device__ __forceinline__ T const *coef_of(Ctx const &c, int m)
+    {
+        return S::kCoefLds ? c.coef + m * N * N : nullptr;
+    }
+
+    // exchange the vector from the layout of phase FROM to that of phase TO through LDS (canonical order,
+    // the exchange swizzle of the forward sweep for both directions)
+    template<int FROM, int TO>
+    static __device__ __forceinline__ void exchange(Ctx const &c, T (&v)[R])
+    {
+        using SW = typename L::SW_EX;
+        lds_barrier();
+#pragma unroll
+        for (int j = 0; j < R; ++j) c.ex[SW::phys(PL::elem(FROM, c.t, j))] = v[j];
+        lds_barrier();
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = c.ex[SW::phys(PL::elem(TO, c.t, j))];
+    }
+
+    // fiber-major store of a vector held in the layout of the phase whose mode is in registers
+    static __device__ __forceinline__ void store_fm(T *dst, T const (&v)[R], int t)
+    {
+#pragma unroll
+        for (int j = 0; j < R; ++j) dst[j * S::RS + t] = v[j];
+    }
+
+    // X chain from phase PH (vector = X_{d-1-PH}) up to phase TO (vector = X_{d-1-TO})
+    template<int PH, int TO>
+    static __device__ __forceinline__ void xchain(Ctx const &c, T (&v)[R])
+    {
+        if constexpr (PH < TO)
+        {
+            int const m = D - 1 - PH;
+            contract<T, PL, false>(v, c.A[m], c.stride, coef_of(c, m));
+            exchange<PH, PH + 1>(c, v);
+            xchain<PH + 1, TO>(c, v);
+        }
+    }
+
+    // dA_m from the fiber-major H (c.ht) and X (xfm); ends with a barrier-free atomic flush
+    static __device__ __forceinline__ void factor_gradient(Ctx const &c, T const *xfm, int m)
+    {
+        using V = typename kronreg::Granule<T>::type;
+        constexpr int G = S::G;
+        int const o = c.t % S::NN;
+        int const sub = c.t / S::NN;
+        int const r = o / N, col = o % N;
+        T const *hp = c.ht + r * S::RS + sub * S::FS;
+        T const *xp = xfm + col * S::RS + sub * S::FS;
+        // four independent partial sums per granule lane keep the FMA pipeline busy
+        constexpr int U = (S::FS / G) % 4 == 0 ? 4 : 1;
+        T part[U][G];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < G; ++i) part[u][i] = T(0);
+#pragma unroll 2
+        for (int f = 0; f < S::FS; f += U * G)
+        {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                V const hv = *reinterpret_cast<V const *>(hp + f + u * G);
+                V const xv = *reinterpret_cast<V const *>(xp + f + u * G);
+#pragma unroll
+                for (int i = 0; i < G; ++i) part[u][i] = fma(hv[i], xv[i], part[u][i]);
+            }
+        }
+        T acc = T(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < G; ++i) acc += part[u][i];
+        if constexpr (S::NSUB > 1)
+        {
+            c.red[sub * S::NN + o] = acc;
+            lds_barrier();
+            if (c.t < S::NN)
+            {
+#pragma unroll
+                for (int s2 = 1; s2 < S::NSUB; ++s2) acc += c.red[s2 * S::NN + o];
+            }
+        }
+        if (c.t < S::NN)
+        {
+            global_ptr<T> g = as_global(c.gA[m]);
+            atomicAdd((T *)(g + r + col * c.stride), acc);
+        }
+    }
+
+    // step t of the H chain (h in the layout of phase d-1-TT); X_TT from the kept x (RECOMPUTE) or the
+    // stored chain (STORE)
+    template<int TT, bool STORE>
+    static __device__ __forceinline__ void hstep(Ctx const &c, T (&h)[R], T const (&xr)[R], bool want_mats,
+                                                 bool want_input)
+    {
+        constexpr int PH = D - 1 - TT;  // phase with mode TT in registers
+        if (want_mats)
+        {
+            T const *xfm;
+            if constexpr (STORE && TT < D - 1)
+            {
+                lds_barrier();  // previous readers of ht are done
+                store_fm(c.ht, h, c.t);
+                xfm = c.xs + TT * S::FM;
+            }
+            else
+            {
+                T v[R];
+#pragma unroll
+                for (int j = 0; j < R; ++j) v[j] = xr[j];
+                xchain<0, PH>(c, v);
+                lds_barrier();  // exchange reads (and previous readers of ht, ex) are done
+                store_fm(c.ex, v, c.t);
+                store_fm(c.ht, h, c.t);
+                xfm = c.ex;
+            }
+            lds_barrier();
+            factor_gradient(c, xfm, TT);
+        }
+        if (TT < D - 1 || want_input) contract<T, PL, true>(h, c.A[TT], c.stride, coef_of(c, TT));
+        if constexpr (TT < D - 1)
+        {
+            exchange<PH, PH - 1>(c, h);
+            hstep<TT + 1, STORE>(c, h, xr, want_mats, want_input);
+        }
+    }
+
+    // STORE: X chain once, X_m for m = d-2 .. 0 kept fiber-major in c.xs
+    template<int PH>
+    static __device__ __forceinline__ void xstore(Ctx const &c, T (&v)[R])
+    {
+        if constexpr (PH < D - 1)
+        {
+            int const m = D - 1 - PH;
+            contract<T, PL, false>(v, c.A[m], c.stride, coef_of(c, m));
+            exchange<PH, PH + 1>(c, v);
+            store_fm(c.xs + (m - 1) * S::FM, v, c.t);  // X_{m-1}, now with mode m-1 in registers
+            xstore<PH + 1>(c, v);
+        }
+    }
+};
+
+template<typename T, typename PL, bool STORE>
+__global__ void __launch_bounds__(PL::P)
+    kron_reg_adjoint_kernel(int nb, int epw, T const *const *__restrict__ mats, int stride,
+                            T const *const *__restrict__ in, T const *const *__restrict__ gout,
+                            T *const *__restrict__ gin, T *const *__restrict__ gmats)
+{
+    using S = AdjShape<T, PL>;
+    using A = Adj<T, PL>;
+    constexpr int R = S::R, P = S::P, D = S::D;
+    __shared__ __attribute__((aligned(16))) T lds[S::lds_elems(STORE)];
+    int const t = threadIdx.x;
+    int const e0 = kronreg::xcd_run_index() * epw;
+    int const e1 = (e0 + epw < nb) ? e0 + epw : nb;
+    if (e0 >= e1) return;
+    bool const want_mats = gmats != nullptr;
+    bool const want_input = gin != nullptr;
+
+    typename A::Ctx c;
+    c.ex = lds;
+    c.ht = lds + S::EX;
+    c.xs = c.ht + S::FM;
+    c.coef = c.xs + (STORE ? (D - 1) * S::FM : 0);
+    c.red = c.coef + S::COEF;
+    c.stride = stride;
+    c.t = t;
+
+    // prefetch registers: x in the phase-0 layout (elements t*R + j), g in the phase-(d-1) layout
+    // (elements j*P + t), staged factors
+    T xpre[R], gpre[R];
+    constexpr int kCoefPer = S::kCoefLds ? kronreg::Coef<T, PL>::kPerThread : 1;
+    T cpre[kCoefPer];
+    auto prefetch = [&](int e) {
+        if (want_mats)
+        {
+            global_ptr<T const> x = as_global(in[e]);
+#pragma unroll
+            for (int j = 0; j < R; ++j) xpre[j] = x[t * R + j];
+        }
+        global_ptr<T const> g = as_global(gout[e]);
+#pragma unroll
+        for (int j = 0; j < R; ++j) gpre[j] = g[j * P + t];
+        if constexpr (S::kCoefLds) kronreg::Coef<T, PL>::load(mats + static_cast<size_t>(e) * D, stride, cpre, t);
+    };
+    prefetch(e0);
+
+    for (int e = e0; e < e1; ++e)
+    {
+        T xr[R], h[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+        {
+            xr[j] = xpre[j];
+            h[j] = gpre[j];
+        }
+        lds_barrier();  // the previous entry is done with every LDS region
+        if constexpr (S::kCoefLds) kronreg::Coef<T, PL>::store(c.coef, cpre, t);
+        if (e + 1 < e1) prefetch(e + 1);
+        c.A = mats + static_cast<size_t>(e) * D;
+        c.gA = want_mats ? gmats + static_cast<size_t>(e) * D : nullptr;
+        if constexpr (STORE)
+        {
+            if (want_mats)
+            {
+                T v[R];
+#pragma unroll
+                for (int j = 0; j < R; ++j) v[j] = xr[j];
+                lds_barrier();  // staged factors visible
+                A::template xstore<0>(c, v);
+            }
+        }
+        lds_barrier();  // staged factors (and stored chain) visible
+        A::template hstep<0, STORE>(c, h, xr, want_mats, want_input);
+        if (want_input)
+        {
+            // h = d x in the phase-0 layout (elements t*R + j): through LDS into the phase-(d-1) layout
+            // (elements j*P + t), so that every atomic instruction covers one contiguous stretch
+            A::template exchange<0, D - 1>(c, h);
+            global_ptr<T> gi = as_global(gin[e]);
+#pragma unroll
+            for (int j = 0; j < R; ++j) atomicAdd((T *)(gi + j * P + t), h[j]);
+        }
+    }
+}
+
+} // namespace kronregadj

@@ -27,6 +27,7 @@
 #include "kron_group.hpp"
 #include "kron_split.hpp"
 #include "kron_reg.hpp"
+#include "kron_reg_adjoint.hpp"
 #include "kronmult_hip.h"
 
 namespace
@@ -412,6 +413,62 @@ hipError_t adjoint_plan(Shape const &s, int elem, int intermediates, size_t max_
     return hipSuccess;
 }
 
+// Register-resident adjoint (kron_reg_adjoint.hpp): one-mode-per-phase plans. STORE keeps d-1 extra
+// vectors in LDS and is offered where that fits comfortably; RECOMPUTE everywhere.
+template<typename T>
+bool reg_adjoint_store_ok(Shape const &s)
+{
+    return (s.d == 3 && s.n == 8) || (s.d == 4 && s.n == 4);
+}
+
+template<typename T>
+bool reg_adjoint_ok(Shape const &s, int intermediates)
+{
+    bool const plan = (s.d == 3 && s.n == 8) || (s.d == 4 && s.n == 4) || (s.d == 4 && s.n == 8);
+    if (!plan) return false;
+    return intermediates != KRONMULT_ADJOINT_STORE || reg_adjoint_store_ok<T>(s);
+}
+
+template<typename T, int D, int N>
+hipError_t reg_adjoint_launch(Shape const &s, bool store, T const *const *mats, T const *const *in,
+                              T const *const *gout, T *const *gin, T *const *gmats, hipStream_t stream)
+{
+    using PL = kronplan::Plan<D, N, 1>;
+    // short runs (about one entry per workgroup at the d=3 n=8 batch size): the adjoint's per-entry chain is
+    // long, so balance beats prefetch depth (measured: 46 us vs 59 us with the forward's run length)
+    int const per_cu = 4 * default_workgroups_per_cu(PL::P);
+    long const slots = static_cast<long>(device_cu_count()) * per_cu;
+    int epw = static_cast<int>((s.nb + slots - 1) / slots);
+    if (epw < 1) epw = 1;
+    long const runs = (s.nb + epw - 1) / epw;
+    long const grid = (runs + 7) / 8 * 8;
+    if (store)
+    {
+        if constexpr ((D == 3 && N == 8) || (D == 4 && N == 4))
+        {
+            hipLaunchKernelGGL((kronregadj::kron_reg_adjoint_kernel<T, PL, true>), dim3(static_cast<unsigned>(grid)),
+                               dim3(PL::P), 0, stream, s.nb, epw, mats, s.stride, in, gout, gin, gmats);
+            return hipGetLastError();
+        }
+        return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL((kronregadj::kron_reg_adjoint_kernel<T, PL, false>), dim3(static_cast<unsigned>(grid)),
+                       dim3(PL::P), 0, stream, s.nb, epw, mats, s.stride, in, gout, gin, gmats);
+    return hipGetLastError();
+}
+
+template<typename T>
+hipError_t reg_adjoint_dispatch(Shape const &s, int intermediates, T const *const *mats, T const *const *in,
+                                T const *const *gout, T *const *gin, T *const *gmats, hipStream_t stream)
+{
+    bool const store = intermediates == KRONMULT_ADJOINT_STORE ||
+                       (intermediates == KRONMULT_ADJOINT_AUTO && reg_adjoint_store_ok<T>(s));
+    if (s.d == 3 && s.n == 8) return reg_adjoint_launch<T, 3, 8>(s, store, mats, in, gout, gin, gmats, stream);
+    if (s.d == 4 && s.n == 4) return reg_adjoint_launch<T, 4, 4>(s, store, mats, in, gout, gin, gmats, stream);
+    if (s.d == 4 && s.n == 8) return reg_adjoint_launch<T, 4, 8>(s, store, mats, in, gout, gin, gmats, stream);
+    return hipErrorInvalidValue;
+}
+
 template<typename T, int N, bool InLds, bool Store>
 hipError_t adjoint_launch_n(Shape const &s, AdjointPlan const &p, T const *const *mats, T const *const *in,
                             T const *const *gout, T *const *gin, T *const *gmats, T *const *ws, hipStream_t stream)
@@ -445,6 +502,8 @@ hipError_t kronmult_adjoint_launch(int d, int n, T const *const *mats, int strid
     if (err != hipSuccess) return err;
     if (nb == 0 || (gin == nullptr && gmats == nullptr)) return hipSuccess;
     if (mats == nullptr || gout == nullptr || (gmats != nullptr && in == nullptr)) return hipErrorInvalidValue;
+    if (reg_adjoint_ok<T>(s, intermediates))
+        return reg_adjoint_dispatch<T>(s, intermediates, mats, in, gout, gin, gmats, stream);
     if (p.ws_elems > 0 && ws == nullptr) return hipErrorInvalidValue;
     switch (n)
     {
