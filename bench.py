@@ -11,13 +11,14 @@ timed region (bracketed by device synchronisation) is one replay, so `ms_per_ste
 throughput without host launch gaps. A separate eager pass records HIP events around every launch on the
 same stream; their mean is the kernel duration used for the roofline.
 
-Printed: one JSON line with GFLOP/s (value), the roofline of the kernel (contract bytes per launch over the
-kernel duration against 8 TB/s HBM; flops against the FP64 / FP32 peak), the HBM traffic per launch
-measured with rocprofv3 counters (profiles/traffic.json), and the reference's CPU implementation timed on
-this host.
+Printed: one JSON line with GFLOP/s (value), the binding roofline of the kernel (`roofline`: compute --
+flops per launch over the kernel duration against the FP64 / FP32 peak -- when the flops at peak take
+longer than the bytes that move at 8 TB/s, else HBM -- contract bytes per launch over the kernel duration),
+both rooflines (`roofline_compute`, `roofline_hbm`), the HBM traffic per launch measured with rocprofv3
+counters (profiles/traffic.json), and the reference's CPU implementation timed on this host.
 
-    python bench.py [--steps K] [--warmup W] [--config c2] [--dtype f64] [--cpu-baseline auto|off]
-                    [--calibrate-copy]
+    python bench.py [--steps K] [--warmup W] [--config c2] [--dtype f64] [--op forward|adjoint|split]
+                    [--intermediates auto|recompute|store] [--cpu-baseline auto|off] [--calibrate-copy]
 """
 import argparse
 import json
@@ -118,6 +119,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--op", default="forward", choices=["forward", "adjoint", "split"],
+                    help="forward: kronmult_batched; adjoint: kronmult_batched_adjoint (input and factor "
+                         "gradients); split: FP64 data through the FP32 matrix cores (error-free splitting)")
+    ap.add_argument("--intermediates", default="auto", choices=["auto", "recompute", "store"],
+                    help="adjoint: how the forward intermediates are obtained")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -143,12 +149,29 @@ def main() -> int:
     dbs = [DeviceBatch(hb, device=str(dev)) for hb in batches]
     stream = torch.cuda.current_stream(dev)
     extra_config = {}
+    if args.op != "forward" and len(dbs) != 1:
+        log("--op adjoint / split take a single-batch workload")
+        return 2
     if len(dbs) == 1:
         hb, db = batches[0], dbs[0]
         kernel = api.kernel_name(hb.d, hb.n, hb.dtype.itemsize, hb.nb)
         step = lambda s: db.run(sync=False, stream=s)
         extra_config = {"d": hb.d, "n": hb.n, "matrix_stride": hb.stride, "nb_batch": hb.nb,
-                        "distinct_outputs": hb.out_count}
+                        "distinct_outputs": hb.out_count, "op": args.op}
+        if args.op == "adjoint":
+            if hb.grad_out_pool is None:  # a cotangent N(0,1) per output, seeded
+                from kronmult993_amd import gen
+                hb.grad_out_pool = gen.normal(0, 90, hb.out_count * hb.size).astype(hb.dtype)
+            db.adjoint_setup(intermediates=args.intermediates)
+            kernel = f"adjoint ({args.intermediates})"
+            step = lambda s: db.run_adjoint(sync=False, stream=s)
+            extra_config["intermediates"] = args.intermediates
+        elif args.op == "split":
+            if args.dtype != "f64":
+                log("--op split takes FP64 data")
+                return 2
+            kernel = "split_fixed3"
+            step = lambda s: db.run_split("fixed3", sync=False, stream=s)
     else:
         from kronmult993_amd.autotune import Autotuner, grouped_call_from_choice
         choice = Autotuner().tune_grouped(dbs)
@@ -202,10 +225,23 @@ def main() -> int:
 
     flops = sum(hb.flops() for hb in batches)
     contract = sum(hb.contract_bytes() for hb in batches)
+    if args.op == "adjoint":
+        # per entry: transposed sweep + factor-gradient contractions + forward chain, 2 flops per term;
+        # bytes: input, cotangent, input gradient read and written, factors, factor gradients read and written
+        s_ = hb.dtype.itemsize
+        per_flop = (6 * hb.d - 2) * hb.n ** (hb.d + 1)
+        per_byte = s_ * (4 * hb.size + 3 * hb.d * hb.n * hb.n) + 8 * (2 + 2 * hb.d)
+        flops, contract = hb.nb * per_flop, hb.nb * per_byte
     gflops = flops / (ms_per_step * 1e-3) / 1e9
     achieved_gbs = contract / (k_avg * 1e-3) / 1e9
     peak_tf = FP64_PEAK_TFLOPS if args.dtype == "f64" else FP32_PEAK_TFLOPS
     achieved_tf = flops / (k_avg * 1e-3) / 1e12
+    traffic_key = f"{args.config}_{args.dtype}" + ("" if args.op == "forward" else f"_{args.op}")
+    traffic = load_traffic(traffic_key, kernel)
+    # binding roofline: the HBM time of the bytes that actually move (measured traffic when recorded, the
+    # per-entry contract bytes otherwise) against the compute time at peak
+    t_hbm = (traffic if traffic is not None else contract) / (HBM_PEAK_GBS * 1e9)
+    t_cmp = flops / (peak_tf * 1e12)
     result = {
         "metric": "batched kron-matvec GFLOP/s & %roofline, d=6 n=4 FP64 vs FP32-split at 1e-10",
         "value": round(gflops, 2),
@@ -221,17 +257,21 @@ def main() -> int:
         "data": "synthetic (seeded counter-based generator, seed 0)",
         "config": dict({"workload": f"{args.config}: {WORKLOADS[args.config]}", "kernel": kernel},
                        **extra_config),
-        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(f"{args.config}_{args.dtype}", kernel)},
-        "roofline_compute": {"bound": "fp64" if args.dtype == "f64" else "fp32", "achieved": round(achieved_tf, 3),
-                             "peak": peak_tf, "unit": "TFLOP/s", "frac": round(achieved_tf / peak_tf, 4)},
+        "roofline": None,  # the binding one of the two below
+        "roofline_hbm": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "roofline_compute": {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": peak_tf,
+                             "unit": "TFLOP/s", "frac": round(achieved_tf / peak_tf, 4), "traffic": traffic,
+                             "note": ("FP64 vector and matrix peak are equal on MI355X (78.6 TFLOP/s); the sweep "
+                                      "runs on the vector FMA units" if args.dtype == "f64" else
+                                      "FP32 peak 157.3 TFLOP/s")},
         "kernel_us": {"avg": round(k_avg * 1e3, 3), "min": round(min(kernel_ms) * 1e3, 3),
                       "max": round(max(kernel_ms) * 1e3, 3)},
         "contract_bytes_per_launch": contract,
         "flops_per_launch": flops,
     }
-    if args.cpu_baseline == "auto":
+    result["roofline"] = result["roofline_compute"] if t_cmp > t_hbm else result["roofline_hbm"]
+    if args.cpu_baseline == "auto" and args.op == "forward":
         result["cpu_baseline"] = cpu_baseline(batches, args.cpu_runs)
     if args.calibrate_copy:
         src = torch.ones(256 * 1024 * 1024 // 8, dtype=torch.float64, device=dev)

@@ -131,3 +131,40 @@ def test_kernel_selection():
     assert api.kernel_name(6, 8, 8, 3903).startswith("fiber_global")
     assert api.kernel_name(2, 100, 8, 1) == "unsupported"
     assert "gfx950" in api.version()
+
+
+def test_extension_entry_points_reject_invalid_arguments_before_any_device_call():
+    lib = _lib.lib()
+    dummy = ctypes.c_void_p(16)
+    # adjoint: n > 8 is unsupported, intermediates out of range, shape errors
+    assert lib.kronmult_batched_adjoint_f64(2, 9, dummy, 9, dummy, dummy, dummy, dummy, None, 3, 0) == 1
+    assert lib.kronmult_batched_adjoint_f64(2, 4, dummy, 4, dummy, dummy, dummy, dummy, None, 3, 7) == 1
+    assert lib.kronmult_batched_adjoint_f32_async(0, 4, dummy, 4, dummy, dummy, dummy, dummy, None, 3, 0,
+                                                  None) == 1
+    assert lib.kronmult_batched_adjoint_workspace(2, 9, 8, 0) == -1
+    assert lib.kronmult_batched_adjoint_workspace(2, 4, 2, 0) == -1
+    assert lib.kronmult_batched_adjoint_f64(3, 4, None, 4, None, None, None, None, None, 0, 0) == 0
+    # FP64 through FP32 matrix cores: n > 8, mode out of range, vector beyond the on-chip size
+    assert lib.kronmult_batched_split_f64(2, 9, dummy, 9, dummy, dummy, 3, 4, None) == 1
+    assert lib.kronmult_batched_split_f64(2, 4, dummy, 4, dummy, dummy, 3, 9, None) == 1
+    assert lib.kronmult_batched_split_f64(5, 8, dummy, 8, dummy, dummy, 3, 4, None) == 1
+    # grouped: strategy out of range, an invalid group, a negative tile length
+    from kronmult993_amd import api
+    good = api.Group(2, 3, 16, 3, 16, 16, None, 4)
+    for groups, opt in (([good], _lib.GroupedOptions(7, 0)),
+                        ([good, api.Group(0, 3, 16, 3, 16, 16, None, 4)], _lib.GroupedOptions(1, 0)),
+                        ([api.Group(2, 3, 16, 3, 16, 16, None, 4, entries_per_tile=-1)], _lib.GroupedOptions(1, 0))):
+        arr = api.grouped_structs(groups, "f64")
+        assert lib.kronmult_batched_grouped_f64(arr, len(groups), ctypes.byref(opt), None) == 1
+    assert lib.kronmult_batched_grouped_f64(api.grouped_structs([], "f64"), 0, None, None) == 0
+
+
+def test_autotune_keys_and_pass_rule():
+    from kronmult993_amd import autotune
+    assert [autotune.bucket(v) for v in (0, 1, 3, 4, 5, 8192, 8193)] == [1, 1, 4, 4, 8, 8192, 16384]
+    assert autotune.key_single("gfx950", "f64", 6, 4, 8000) == "gfx950|f64|d6n4|nb8192"
+    # entries per pass of the persistent grouped launch: 256 threads / n^(d-1) fibers, within 64 KiB of LDS
+    assert autotune.entries_per_pass(2, 8, 8) == 32
+    assert autotune.entries_per_pass(3, 6, 8) == 7
+    assert autotune.entries_per_pass(4, 4, 8) == 4
+    assert autotune.entries_per_pass(6, 4, 8) == 1
