@@ -250,16 +250,17 @@ struct Adj
     }
 };
 
-// Waves per SIMD the register allocator must allow: the d=3 n=8 FP64 kernel needs a few registers more than
-// three waves' share, and one more resident wave per SIMD is worth more than the spare registers.
-template<typename T, typename PL>
+// Waves per SIMD the register allocator must allow: the d=3 n=8 FP64 recompute kernel needs a few registers
+// more than three waves' share, and one more resident wave per SIMD is worth more than the spare registers
+// (the stored-chain kernel is held to two waves by its LDS anyway).
+template<typename T, typename PL, bool STORE>
 constexpr int adjoint_waves_per_simd()
 {
-    return (PL::kD == 3 && PL::kN == 8 && sizeof(T) == 8) ? 3 : 1;
+    return (!STORE && PL::kD == 3 && PL::kN == 8 && sizeof(T) == 8) ? 3 : 1;
 }
 
 template<typename T, typename PL, bool STORE>
-__global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(adjoint_waves_per_simd<T, PL>())))
+__global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(adjoint_waves_per_simd<T, PL, STORE>())))
     kron_reg_adjoint_kernel(int nb, int epw, T const *const *__restrict__ mats, int stride,
                             T const *const *__restrict__ in, T const *const *__restrict__ gout,
                             T *const *__restrict__ gin, T *const *__restrict__ gmats)
