@@ -79,7 +79,7 @@ class KronmultFunction(_torch().autograd.Function):
         mat_pool, in_pool = mat_pool.contiguous(), in_pool.contiguous()
         out = out_init.detach().clone().contiguous()
         dt = _dtype_name(out)
-        uses_ws = api.kernel_name(L.d, L.n, out.element_size(), L.nb).startswith("fiber_global")
+        uses_ws = api.kernel_name(L.d, L.n, out.element_size(), L.nb).startswith(("fiber_global", "multipass"))
         ws, ws_ptrs = _workspace(out, L.size if uses_ws else 0, L.nb)  # kept alive across the call
         api.kronmult_batched(L.d, L.n, _ptrs(mat_pool, L.mat_off), L.stride, _ptrs(in_pool, L.in_off),
                              _ptrs(out, L.out_off), ws_ptrs, L.nb, dtype=dt)

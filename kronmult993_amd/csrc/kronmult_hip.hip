@@ -25,6 +25,7 @@
 
 #include "kron_adjoint.hpp"
 #include "kron_group.hpp"
+#include "kron_pass.hpp"
 #include "kron_split.hpp"
 #include "kron_reg.hpp"
 #include "kron_reg_adjoint.hpp"
@@ -293,6 +294,100 @@ hipError_t fiber_dispatch_n(Shape const &s, Launch &L, T const *const *mats, T c
     }
 }
 
+// ---- multi-pass family (vectors beyond the on-chip size, n <= 8) ---------------------------------------
+constexpr int kPassMaxN = 8;
+
+// Largest power of n not above `limit` (at least 1).
+long pow_floor(int n, long limit)
+{
+    long p = 1;
+    while (p * n <= limit) p *= n;
+    return p;
+}
+
+template<typename T, int N>
+hipError_t pass_run(Shape const &s, Launch &L, T const *const *mats, T const *const *in, T *const *out,
+                    T *const *ws, hipStream_t stream, bool run)
+{
+    long const E = kronpass::kTileBytes / static_cast<long>(sizeof(T));  // tile elements
+    int gmax = 1;
+    for (long p = N; p * N <= E; p *= N) ++gmax;
+    int const passes = (s.d + gmax - 1) / gmax;
+    L.family = KRONMULT_FAMILY_FIBER;
+    L.block = kronpass::kThreads;
+    L.epw = 1;
+    L.workspace = true;
+    L.resident = 0;
+    int b = s.d;  // groups from the fastest: [b - g, b)
+    for (int q = 0; q < passes; ++q)
+    {
+        int const g = s.d / passes + (q < s.d % passes ? 1 : 0);
+        int const a = b - g;
+        long ng = 1;
+        for (int i = 0; i < g; ++i) ng *= N;
+        long inner = 1;
+        for (int i = b; i < s.d; ++i) inner *= N;
+        long outer = 1;
+        for (int i = 0; i < a; ++i) outer *= N;
+        long TI = pow_floor(N, inner < 16 ? inner : 16);
+        while (TI > 1 && ng * TI > E) TI /= N;
+        long TO = pow_floor(N, E / (ng * TI));
+        if (TO > outer) TO = outer;
+        kronpass::PassArgs P{};
+        P.d = s.d;
+        P.a = a;
+        P.g = g;
+        P.ng = static_cast<int>(ng);
+        P.inner = static_cast<int>(inner);
+        P.TO = static_cast<int>(TO);
+        P.TI = static_cast<int>(TI);
+        P.tiles_i = static_cast<int>(inner / TI);
+        P.tiles_per_entry = static_cast<int>((outer / TO) * (inner / TI));
+        P.first = q == 0;
+        P.last = q == passes - 1;
+        long const grid = static_cast<long>(s.nb) * P.tiles_per_entry;
+        size_t const lds = static_cast<size_t>(TO * ng * TI) * sizeof(T);
+        if (q == 0)
+        {
+            L.grid = grid;
+            L.lds = lds;
+        }
+        if (grid > INT_MAX) return hipErrorInvalidValue;
+        if (run && grid > 0)
+        {
+            hipLaunchKernelGGL((kronpass::kron_pass_kernel<T, N>), dim3(static_cast<unsigned>(grid)),
+                               dim3(kronpass::kThreads), lds, stream, P, mats, s.stride, in, ws, out);
+            hipError_t err = hipGetLastError();
+            if (err != hipSuccess) return err;
+        }
+        b = a;
+    }
+    return hipSuccess;
+}
+
+template<typename T>
+hipError_t pass_dispatch(Shape const &s, Launch &L, T const *const *mats, T const *const *in, T *const *out,
+                         T *const *ws, hipStream_t stream, bool run)
+{
+    switch (s.n)
+    {
+    case 2: return pass_run<T, 2>(s, L, mats, in, out, ws, stream, run);
+    case 3: return pass_run<T, 3>(s, L, mats, in, out, ws, stream, run);
+    case 4: return pass_run<T, 4>(s, L, mats, in, out, ws, stream, run);
+    case 5: return pass_run<T, 5>(s, L, mats, in, out, ws, stream, run);
+    case 6: return pass_run<T, 6>(s, L, mats, in, out, ws, stream, run);
+    case 7: return pass_run<T, 7>(s, L, mats, in, out, ws, stream, run);
+    case 8: return pass_run<T, 8>(s, L, mats, in, out, ws, stream, run);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template<typename T>
+bool use_passes(Shape const &s)
+{
+    return !fiber_in_lds<T>(s) && s.n >= 2 && s.n <= kPassMaxN;
+}
+
 template<typename T>
 hipError_t fiber_dispatch(Shape const &s, Launch &L, T const *const *mats, T const *const *in, T *const *out,
                           T *const *ws, hipStream_t stream, bool run)
@@ -300,6 +395,7 @@ hipError_t fiber_dispatch(Shape const &s, Launch &L, T const *const *mats, T con
     if (s.n > kMaxFiber) return hipErrorInvalidValue;
     if (fiber_in_lds<T>(s)) return fiber_dispatch_n<T, true>(s, L, mats, in, out, ws, stream, run);
     if (run && ws == nullptr) return hipErrorInvalidValue;
+    if (use_passes<T>(s)) return pass_dispatch<T>(s, L, mats, in, out, ws, stream, run);
     return fiber_dispatch_n<T, false>(s, L, mats, in, out, ws, stream, run);
 }
 
@@ -360,6 +456,7 @@ char const *kernel_name(int d, int n, int nb)
     case KRONMULT_FAMILY_REGISTER: return f64 ? "reg_sweep_f64" : "reg_sweep_f32";
     case KRONMULT_FAMILY_FIBER:
         if (fiber_in_lds<T>(s)) return f64 ? "fiber_lds_f64" : "fiber_lds_f32";
+        if (use_passes<T>(s)) return f64 ? "multipass_f64" : "multipass_f32";
         return f64 ? "fiber_global_f64" : "fiber_global_f32";
     default: return "unsupported";
     }

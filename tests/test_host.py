@@ -128,7 +128,8 @@ def test_kernel_selection():
     assert api.kernel_name(6, 4, 8, 8192) == "reg_sweep_f64"
     assert api.kernel_name(4, 8, 4, 16384) == "reg_sweep_f32"
     assert api.kernel_name(3, 6, 8, 6000).startswith("fiber_lds")
-    assert api.kernel_name(6, 8, 8, 3903).startswith("fiber_global")
+    assert api.kernel_name(6, 8, 8, 3903) == "multipass_f64"
+    assert api.kernel_name(3, 21, 8, 6) == "fiber_global_f64"
     assert api.kernel_name(2, 100, 8, 1) == "unsupported"
     assert "gfx950" in api.version()
 
