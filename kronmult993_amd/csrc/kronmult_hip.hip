@@ -305,13 +305,43 @@ long pow_floor(int n, long limit)
     return p;
 }
 
+// TI: consecutive inner indices per tile, a power of n up to 16 (1 when the group holds the fastest mode)
+template<int N>
+constexpr int pass_ti()
+{
+    int p = 1;
+    while (p * N <= 16) p *= N;
+    return p;
+}
+
+template<typename T, int N, int G, int TI>
+hipError_t pass_launch(kronpass::PassArgs const &P, long grid, size_t lds, T const *const *mats, int stride,
+                       T const *const *in, T *const *out, T *const *ws, hipStream_t stream)
+{
+    hipLaunchKernelGGL((kronpass::kron_pass_kernel<T, N, G, TI>), dim3(static_cast<unsigned>(grid)),
+                       dim3(kronpass::kThreads), lds, stream, P, mats, stride, in, ws, out);
+    return hipGetLastError();
+}
+
+template<typename T, int N, int G>
+hipError_t pass_launch_ti(kronpass::PassArgs const &P, int ti, long grid, size_t lds, T const *const *mats,
+                          int stride, T const *const *in, T *const *out, T *const *ws, hipStream_t stream)
+{
+    if (ti == 1) return pass_launch<T, N, G, 1>(P, grid, lds, mats, stride, in, out, ws, stream);
+    return pass_launch<T, N, G, pass_ti<N>()>(P, grid, lds, mats, stride, in, out, ws, stream);
+}
+
+constexpr int kPassMaxGroup = 4;  // modes per pass (compile-time group sizes 1..4)
+constexpr int kPassLastRun = 8;   // entries per workgroup in the last pass (register sums of equal outputs)
+
 template<typename T, int N>
 hipError_t pass_run(Shape const &s, Launch &L, T const *const *mats, T const *const *in, T *const *out,
                     T *const *ws, hipStream_t stream, bool run)
 {
     long const E = kronpass::kTileBytes / static_cast<long>(sizeof(T));  // tile elements
-    int gmax = 1;
-    for (long p = N; p * N <= E; p *= N) ++gmax;
+    constexpr int TIW = pass_ti<N>();
+    int gmax = 1;  // the tile must hold one sub-tensor times TIW inner indices
+    for (long p = N; p * N * TIW <= E && gmax < kPassMaxGroup; p *= N) ++gmax;
     int const passes = (s.d + gmax - 1) / gmax;
     L.family = KRONMULT_FAMILY_FIBER;
     L.block = kronpass::kThreads;
@@ -329,23 +359,21 @@ hipError_t pass_run(Shape const &s, Launch &L, T const *const *mats, T const *co
         for (int i = b; i < s.d; ++i) inner *= N;
         long outer = 1;
         for (int i = 0; i < a; ++i) outer *= N;
-        long TI = pow_floor(N, inner < 16 ? inner : 16);
-        while (TI > 1 && ng * TI > E) TI /= N;
+        int const TI = inner == 1 ? 1 : TIW;
         long TO = pow_floor(N, E / (ng * TI));
         if (TO > outer) TO = outer;
         kronpass::PassArgs P{};
         P.d = s.d;
         P.a = a;
-        P.g = g;
-        P.ng = static_cast<int>(ng);
         P.inner = static_cast<int>(inner);
         P.TO = static_cast<int>(TO);
-        P.TI = static_cast<int>(TI);
         P.tiles_i = static_cast<int>(inner / TI);
         P.tiles_per_entry = static_cast<int>((outer / TO) * (inner / TI));
         P.first = q == 0;
         P.last = q == passes - 1;
-        long const grid = static_cast<long>(s.nb) * P.tiles_per_entry;
+        P.nb = s.nb;
+        P.epw = P.last ? kPassLastRun : 1;
+        long const grid = static_cast<long>((s.nb + P.epw - 1) / P.epw) * P.tiles_per_entry;
         size_t const lds = static_cast<size_t>(TO * ng * TI) * sizeof(T);
         if (q == 0)
         {
@@ -355,9 +383,14 @@ hipError_t pass_run(Shape const &s, Launch &L, T const *const *mats, T const *co
         if (grid > INT_MAX) return hipErrorInvalidValue;
         if (run && grid > 0)
         {
-            hipLaunchKernelGGL((kronpass::kron_pass_kernel<T, N>), dim3(static_cast<unsigned>(grid)),
-                               dim3(kronpass::kThreads), lds, stream, P, mats, s.stride, in, ws, out);
-            hipError_t err = hipGetLastError();
+            hipError_t err;
+            switch (g)
+            {
+            case 1: err = pass_launch_ti<T, N, 1>(P, TI, grid, lds, mats, s.stride, in, out, ws, stream); break;
+            case 2: err = pass_launch_ti<T, N, 2>(P, TI, grid, lds, mats, s.stride, in, out, ws, stream); break;
+            case 3: err = pass_launch_ti<T, N, 3>(P, TI, grid, lds, mats, s.stride, in, out, ws, stream); break;
+            default: err = pass_launch_ti<T, N, 4>(P, TI, grid, lds, mats, s.stride, in, out, ws, stream); break;
+            }
             if (err != hipSuccess) return err;
         }
         b = a;
