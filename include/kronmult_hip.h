@@ -153,7 +153,9 @@ long long kronmult_batched_adjoint_workspace(int matrix_count, int matrix_size, 
  * Strategies (options->strategy; options may be NULL = AUTO with defaults):
  *   KRONMULT_GROUPED_PERSISTENT  one persistent launch over every group with n <= 8 whose vector and
  *                                factors fit in 64 KiB of LDS; `workgroups_per_cu` sizes its grid (0 =
- *                                default 2) and each group's `entries_per_tile` its tiles (0 = one pass);
+ *                                default 2) and each group's `entries_per_tile` its tiles (0 = one pass of
+ *                                min(256 / n^(d-1), what fits) entries; a shorter tile also shortens the pass,
+ *                                which lowers the LDS per workgroup);
  *                                any other group gets a launch of its own, as below;
  *   KRONMULT_GROUPED_SEPARATE    one ordinary launch per group, shaped by that group's `tuning`;
  *   KRONMULT_GROUPED_AUTO        currently the persistent strategy.

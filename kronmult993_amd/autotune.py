@@ -10,8 +10,9 @@ Choices:
   * one batch: the kernel family (register-resident sweep where a plan exists, fiber sweep) and, for the
     register family, workgroups per CU;
   * several batches (a grouped call): the persistent launch -- workgroups per CU and per-group tile length
-    (entries per tile, a multiple of the entries the launch processes in one pass) -- against one launch
-    per group with each group's own best shape.
+    (entries per tile, from a quarter of the entries the launch processes in one pass, which also shrinks
+    the pass and its LDS, to four passes) -- against one launch per group with each group's own best
+    shape.
 """
 from __future__ import annotations
 
@@ -153,9 +154,9 @@ class Autotuner:
         best_t = self._time_us(lambda: api.kronmult_batched_grouped(groups, dtype=dt, strategy="separate",
                                                                     structs=structs))
         # persistent launch: grid and tile length
-        for wpc in (1, 2, 4, 8):
-            for mult in (1, 2, 4, 8):
-                epw = [entries_per_pass(db.d, db.n, db.itemsize) * mult for db in dbs]
+        for wpc in (4, 8, 16, 32):
+            for mult in (0.25, 0.5, 1, 2, 4):
+                epw = [max(1, int(entries_per_pass(db.d, db.n, db.itemsize) * mult)) for db in dbs]
                 groups = [db.group(tuning=t, entries_per_tile=e) for db, t, e in zip(dbs, tun_clean, epw)]
                 structs = api.grouped_structs(groups, dt)
                 t = self._time_us(lambda: api.kronmult_batched_grouped(

@@ -741,6 +741,8 @@ hipError_t kronmult_grouped_launch(GroupT const *groups, int count, kronmult_gro
         int E = krongroup::kThreads / D.nfib;
         if (E < 1) E = 1;
         while (E > 1 && E * per_entry > static_cast<long>(kGroupLdsBudget)) --E;
+        // a tile shorter than one pass shrinks the pass (and the LDS the launch needs, so more workgroups fit)
+        if (g.entries_per_tile > 0 && g.entries_per_tile < E) E = g.entries_per_tile;
         D.E = E;
         D.epw = g.entries_per_tile > 0 ? g.entries_per_tile : E;
         D.mats = g.matrix_list_batched;
