@@ -202,11 +202,11 @@ struct Launch
 // ---- register family --------------------------------------------------------------------------------
 // grid = CUs x workgroups per CU (a multiple of 8, so every XCD gets the same number of runs); each
 // workgroup walks ceil(nb / grid) consecutive entries. The default number of workgroups per CU was
-// measured on MI355X (launch-shape sweeps of the d=6 n=4, d=3 n=8 and d=4 n=8 batches): about 16
-// wavefronts per CU for 4- and 1-wavefront workgroups, a single 8-wavefront workgroup.
+// measured on MI355X (launch-shape sweeps of the d=6 n=4, d=3 n=8 and d=4 n=8 batches): 4 workgroups of 4
+// or 8 wavefronts per CU (the runs are then short enough to balance), 8 one-wavefront workgroups.
 constexpr int default_workgroups_per_cu(int threads)
 {
-    return threads >= 512 ? 1 : threads >= 256 ? 4 : threads >= 128 ? 4 : 8;
+    return threads >= 512 ? 4 : threads >= 256 ? 4 : threads >= 128 ? 4 : 8;
 }
 template<typename T, int D, int N>
 hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T const *const *mats,
