@@ -110,7 +110,8 @@ int kronmult_hip_launch_info(int matrix_count, int matrix_size, int element_size
  * intermediates selects how the forward sweep's intermediate vectors are obtained:
  *   KRONMULT_ADJOINT_RECOMPUTE  rebuilt from the input for every factor (two n^d vectors of scratch);
  *   KRONMULT_ADJOINT_STORE      computed once and kept (d vectors of scratch);
- *   KRONMULT_ADJOINT_AUTO       STORE when its scratch fits in on-chip memory, else RECOMPUTE.
+ *   KRONMULT_ADJOINT_AUTO       the faster of the two where the library has measured it, else STORE when
+ *                               its scratch fits in on-chip memory, else RECOMPUTE.
  * Scratch is on-chip when it fits; otherwise workspace_batched[k] must point to
  * kronmult_batched_adjoint_workspace(d, n, sizeof(T), intermediates) elements per entry (0 = not read).
  */

@@ -14,8 +14,9 @@
 //   * dA_t: when mode t is in registers, H_t and X_t are written to LDS fiber-major (row r = index of mode
 //     t, column = fiber, rows padded by one 16-byte granule so different rows sit in different banks); then
 //     thread (r, c, subset s) sums h_f[r] * x_f[c] over its subset of fibers (16-byte loads, broadcast
-//     across the threads that share r or c), subsets are reduced through LDS and one atomic per factor
-//     element is added to the gradient.
+//     across the threads that share r or c) -- for FP64 with n = 4 or 8 the same sums run on the matrix
+//     cores instead --, partial sums are reduced through LDS and one atomic per factor element is added
+//     to the gradient.
 // Gradients of shared storage are summed with hardware atomic adds, as in the forward epilogue.
 #pragma once
 
@@ -197,6 +198,73 @@ struct Adj
         }
     }
 
+    // FP64, n = 4 or 8: the same sums on the matrix cores. V_MFMA_F64_4X4X4_4B_F64 computes four 4 x 4
+    // blocks, K = 4 fibers per instruction; lane 16*kb + 4*jb + i supplies A_jb[i][kb] and B_jb[kb][i], and
+    // block jb's result D_jb[i][s] lands in lane 16*i + 4*jb + s (measured on gfx950). For n = 8 the four
+    // blocks are the 2 x 2 quarters of dA; for n = 4 they take a quarter of the fibers each and are summed
+    // across lanes afterwards. Wavefront w takes fibers [w*FW, (w+1)*FW); partial results meet in LDS.
+    static constexpr bool kMfmaGradient = sizeof(T) == 8 && (N == 4 || N == 8);
+
+    static __device__ __forceinline__ void factor_gradient_mfma(Ctx const &c, T const *xfm, int m)
+    {
+        constexpr int NW = P / 64;
+        constexpr int FW = S::NFIB / NW;
+        static_assert(FW % (N == 8 ? 8 : 32) == 0, "fibers per wavefront must fill the K-steps");
+        int const lane = c.t & 63, wave = c.t >> 6;
+        int const kb = lane >> 4, jb = (lane >> 2) & 3, lo = lane & 3;
+        double acc0 = 0.0, acc1 = 0.0;  // two chains: even and odd K-steps
+        int r, col;
+        if constexpr (N == 8)
+        {
+            int const rh = jb >> 1, ch = jb & 1;
+            T const *hp = c.ht + (4 * rh + lo) * S::RS + wave * FW + kb;
+            T const *xp = xfm + (4 * ch + lo) * S::RS + wave * FW + kb;
+#pragma unroll 4
+            for (int f = 0; f < FW; f += 8)
+            {
+                acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(hp[f], xp[f], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(hp[f + 4], xp[f + 4], acc1, 0, 0, 0);
+            }
+            r = 4 * rh + kb;  // kb is the output row index i of the D layout
+            col = 4 * ch + lo;
+        }
+        else
+        {
+            constexpr int FB = FW / 4;  // fibers per block
+            T const *hp = c.ht + lo * S::RS + wave * FW + jb * FB + kb;
+            T const *xp = xfm + lo * S::RS + wave * FW + jb * FB + kb;
+#pragma unroll 4
+            for (int f = 0; f < FB; f += 8)
+            {
+                acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(hp[f], xp[f], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(hp[f + 4], xp[f + 4], acc1, 0, 0, 0);
+            }
+            r = kb;
+            col = lo;
+        }
+        double acc = acc0 + acc1;
+        if constexpr (N == 4)
+        {
+            acc += __shfl_xor(acc, 4);
+            acc += __shfl_xor(acc, 8);
+        }
+        if constexpr (NW > 1)
+        {
+            c.red[wave * 64 + lane] = acc;
+            lds_barrier();
+            if (wave == 0)
+            {
+#pragma unroll
+                for (int w = 1; w < NW; ++w) acc += c.red[w * 64 + lane];
+            }
+        }
+        if (wave == 0 && (N == 8 || jb == 0))
+        {
+            global_ptr<T> g = as_global(c.gA[m]);
+            atomicAdd((T *)(g + r + col * c.stride), acc);
+        }
+    }
+
     // step t of the H chain (h in the layout of phase d-1-TT); X_TT from the kept x (RECOMPUTE) or the
     // stored chain (STORE)
     template<int TT, bool STORE>
@@ -225,7 +293,10 @@ struct Adj
                 xfm = c.ex;
             }
             lds_barrier();
-            factor_gradient(c, xfm, TT);
+            if constexpr (kMfmaGradient)
+                factor_gradient_mfma(c, xfm, TT);
+            else
+                factor_gradient(c, xfm, TT);
         }
         if (TT < D - 1 || want_input) contract<T, PL, true>(h, c.A[TT], c.stride, coef_of(c, TT));
         if constexpr (TT < D - 1)
