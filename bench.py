@@ -31,6 +31,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# OpenMP placement for the CPU baseline (read when the reference's OpenMP runtime starts): threads pinned to
+# cores, packed; the caller's settings win
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
@@ -91,11 +96,13 @@ def cpu_baseline(hbs, runs: int):
         log(f"[cpu] run {r} {'(warm-up) ' if r == 0 else ''}{(t1 - t0) * 1e3:.1f} ms on {threads} threads")
     med = statistics.median(times)
     flops = sum(hb.flops() for hb in hbs)
+    placement = f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} OMP_PLACES={os.environ.get('OMP_PLACES')}"
     names = "+".join(hb.name for hb in hbs)
     nb = sum(hb.nb for hb in hbs)
     return {"value": round(flops / med / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": kind,
             "sample": (f"full {names} batch ({nb} entries, {flops / 1e9:.3f} GFLOP), private input "
-                       f"copies, 1 warm-up + {runs} runs, median {med * 1e3:.1f} ms, min {min(times) * 1e3:.1f} ms")}
+                       f"copies, 1 warm-up + {runs} runs, median {med * 1e3:.1f} ms, min {min(times) * 1e3:.1f} ms, "
+                       f"{placement}")}
 
 
 def load_traffic(key: str, kernel: str):
@@ -125,7 +132,7 @@ def main() -> int:
     ap.add_argument("--intermediates", default="auto", choices=["auto", "recompute", "store"],
                     help="adjoint: how the forward intermediates are obtained")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
-    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, copy a 256 MiB device buffer (reference kernel for byte counters)")
     args = ap.parse_args()
