@@ -222,6 +222,29 @@ int kronmult_batched_split_f64(int matrix_count, int matrix_size, double const *
                                int split_mode, void *stream);
 
 /*
+ * Pointer arrays of an element-pair operator batch, built on the device (no counterpart in the reference:
+ * its caller, ASGarD, assembles nb_batch*(d+2) pointers on the host before every apply). For row elements
+ * row_ids[0..nrows) and column elements col_ids[0..ncols), entry k = a*ncols + b is
+ *   factor t (0 <= t < d): the n x n block of coefficient matrix coeff[t] (column-major, leading dimension
+ *                          coeff_ld >= n*base) at row n*digit_t(row_ids[a]), column n*digit_t(col_ids[b]),
+ *                          with digit_t(id) = (id / base^t) mod base;
+ *   input  inputs[b] (one n^d vector per column element), output outputs[a] (one per row element).
+ * Writes matrix_list_batched[k*d + t], input_batched[k] and output_batched[k] for the nrows*ncols entries
+ * (to be used with matrix_stride = coeff_ld). Every array is device memory, including coeff, row_ids,
+ * col_ids, inputs and outputs. Stream-ordered, no allocation.
+ */
+int kronmult_build_element_batch_f64(int matrix_count, int matrix_size, int base, double const *const *coeff,
+                                     int coeff_ld, int const *row_ids, int nrows, int const *col_ids, int ncols,
+                                     double *const *inputs, double *const *outputs,
+                                     double const **matrix_list_batched, double **input_batched,
+                                     double **output_batched, void *stream);
+int kronmult_build_element_batch_f32(int matrix_count, int matrix_size, int base, float const *const *coeff,
+                                     int coeff_ld, int const *row_ids, int nrows, int const *col_ids, int ncols,
+                                     float *const *inputs, float *const *outputs,
+                                     float const **matrix_list_batched, float **input_batched,
+                                     float **output_batched, void *stream);
+
+/*
  * Library version string and the name of the kernel the dispatcher would pick for (d, n, element size in
  * bytes, nb_batch); both point to static storage. No reference counterpart (diagnostics only).
  */

@@ -66,6 +66,8 @@ SIGNATURES = {
     "kronmult_batched_grouped_f64": (_I, [ctypes.POINTER(GroupF64), _I, ctypes.POINTER(GroupedOptions), _P]),
     "kronmult_batched_grouped_f32": (_I, [ctypes.POINTER(GroupF32), _I, ctypes.POINTER(GroupedOptions), _P]),
     "kronmult_batched_split_f64": (_I, [_I, _I, _P, _I, _P, _P, _I, _I, _P]),
+    "kronmult_build_element_batch_f64": (_I, [_I, _I, _I, _P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P]),
+    "kronmult_build_element_batch_f32": (_I, [_I, _I, _I, _P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P]),
     "kronmult_hip_version": (ctypes.c_char_p, []),
     "kronmult_hip_kernel_name": (ctypes.c_char_p, [_I, _I, _I, _I]),
 }

@@ -208,6 +208,18 @@ def kronmult_batched_split(matrix_count: int, matrix_size: int, matrix_list_batc
         raise KronmultError(rc, "kronmult_batched_split_f64")
 
 
+def build_element_batch(matrix_count: int, matrix_size: int, base: int, coeff, coeff_ld: int, row_ids, nrows: int,
+                        col_ids, ncols: int, inputs, outputs, matrix_list_batched, input_batched, output_batched,
+                        *, dtype, stream=None) -> None:
+    """Device-side construction of an element-pair operator batch's pointer arrays (stream-ordered)."""
+    f = getattr(lib(), "kronmult_build_element_batch_" + _suffix(dtype))
+    rc = f(matrix_count, matrix_size, base, _addr(coeff), coeff_ld, _addr(row_ids), nrows, _addr(col_ids), ncols,
+           _addr(inputs), _addr(outputs), _addr(matrix_list_batched), _addr(input_batched), _addr(output_batched),
+           _stream_handle(stream))
+    if rc != HIP_SUCCESS:
+        raise KronmultError(rc, "kronmult_build_element_batch")
+
+
 def kernel_name(matrix_count: int, matrix_size: int, element_size: int, nb_batch: int) -> str:
     return lib().kronmult_hip_kernel_name(matrix_count, matrix_size, element_size, nb_batch).decode()
 

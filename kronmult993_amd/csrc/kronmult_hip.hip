@@ -819,6 +819,46 @@ hipError_t kronmult_split_launch(int d, int n, double const *const *mats, int st
     }
 }
 
+// ---- batch builder: element-pair operator batches ----------------------------------------------------
+template<typename T>
+__global__ void __launch_bounds__(256)
+    build_element_batch_kernel(int d, int n, int base, T const *const *__restrict__ coeff, int ld,
+                               int const *__restrict__ row_ids, int const *__restrict__ col_ids, int ncols,
+                               long nb, T *const *__restrict__ inputs, T *const *__restrict__ outputs,
+                               T const **__restrict__ mats, T **__restrict__ in, T **__restrict__ out)
+{
+    long const k = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    int const a = static_cast<int>(k / ncols);
+    int const b = static_cast<int>(k - static_cast<long>(a) * ncols);
+    int ra = row_ids[a], cb = col_ids[b];
+    for (int t = 0; t < d; ++t)
+    {
+        int const dr = ra % base, dc = cb % base;
+        ra /= base;
+        cb /= base;
+        mats[k * d + t] = coeff[t] + static_cast<long>(n) * dr + static_cast<long>(n) * dc * ld;
+    }
+    in[k] = inputs[b];
+    out[k] = outputs[a];
+}
+
+template<typename T>
+hipError_t build_element_batch(int d, int n, int base, T const *const *coeff, int ld, int const *row_ids,
+                               int nrows, int const *col_ids, int ncols, T *const *inputs, T *const *outputs,
+                               T const **mats, T **in, T **out, hipStream_t stream)
+{
+    if (d < 1 || n < 1 || base < 1 || nrows < 0 || ncols < 0) return hipErrorInvalidValue;
+    if (static_cast<long>(ld) < static_cast<long>(n) * base) return hipErrorInvalidValue;
+    long const nb = static_cast<long>(nrows) * ncols;
+    if (nb > INT_MAX) return hipErrorInvalidValue;
+    if (nb == 0) return hipSuccess;
+    if (!coeff || !row_ids || !col_ids || !inputs || !outputs || !mats || !in || !out) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(build_element_batch_kernel<T>, dim3(static_cast<unsigned>((nb + 255) / 256)), dim3(256), 0,
+                       stream, d, n, base, coeff, ld, row_ids, col_ids, ncols, nb, inputs, outputs, mats, in, out);
+    return hipGetLastError();
+}
+
 } // namespace
 
 extern "C"
@@ -975,6 +1015,30 @@ int kronmult_batched_split_f64(int matrix_count, int matrix_size, double const *
     return static_cast<int>(kronmult_split_launch(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
                                                   input_batched, output_batched, nb_batch, split_mode,
                                                   static_cast<hipStream_t>(stream)));
+}
+
+int kronmult_build_element_batch_f64(int matrix_count, int matrix_size, int base, double const *const *coeff,
+                                     int coeff_ld, int const *row_ids, int nrows, int const *col_ids, int ncols,
+                                     double *const *inputs, double *const *outputs,
+                                     double const **matrix_list_batched, double **input_batched,
+                                     double **output_batched, void *stream)
+{
+    return static_cast<int>(build_element_batch<double>(matrix_count, matrix_size, base, coeff, coeff_ld, row_ids,
+                                                        nrows, col_ids, ncols, inputs, outputs, matrix_list_batched,
+                                                        input_batched, output_batched,
+                                                        static_cast<hipStream_t>(stream)));
+}
+
+int kronmult_build_element_batch_f32(int matrix_count, int matrix_size, int base, float const *const *coeff,
+                                     int coeff_ld, int const *row_ids, int nrows, int const *col_ids, int ncols,
+                                     float *const *inputs, float *const *outputs,
+                                     float const **matrix_list_batched, float **input_batched,
+                                     float **output_batched, void *stream)
+{
+    return static_cast<int>(build_element_batch<float>(matrix_count, matrix_size, base, coeff, coeff_ld, row_ids,
+                                                       nrows, col_ids, ncols, inputs, outputs, matrix_list_batched,
+                                                       input_batched, output_batched,
+                                                       static_cast<hipStream_t>(stream)));
 }
 
 char const *kronmult_hip_version(void)

@@ -169,3 +169,14 @@ def test_autotune_keys_and_pass_rule():
     assert autotune.entries_per_pass(3, 6, 8) == 7
     assert autotune.entries_per_pass(4, 4, 8) == 4
     assert autotune.entries_per_pass(6, 4, 8) == 1
+
+
+def test_builder_rejects_invalid_arguments_before_any_device_call():
+    lib = _lib.lib()
+    p = ctypes.c_void_p(16)
+    args = lambda d, n, base, ld, nr, nc: (d, n, base, p, ld, p, nr, p, nc, p, p, p, p, p, None)
+    assert lib.kronmult_build_element_batch_f64(*args(0, 4, 16, 64, 2, 2)) == 1
+    assert lib.kronmult_build_element_batch_f64(*args(6, 4, 16, 63, 2, 2)) == 1  # ld < n * base
+    assert lib.kronmult_build_element_batch_f64(*args(6, 4, 0, 64, 2, 2)) == 1
+    assert lib.kronmult_build_element_batch_f32(*args(6, 4, 16, 64, -1, 2)) == 1
+    assert lib.kronmult_build_element_batch_f64(*args(6, 4, 16, 64, 0, 5)) == 0  # empty batch
