@@ -49,6 +49,8 @@ WORKLOADS = {
            "factors, 4 consecutive entries per output, one grouped call (strategy and shapes autotuned)"),
     "c4": "d=4 n=8 stride=8, nb=16384, factors U[-1,1]*10^U[-2,2], 1024 outputs x 16 contributions",
     "c5": "d=3 n=8 stride=8, nb=8192, 256 shared factor triples, 2048 outputs x 4 contributions",
+    "ref_large": ("the reference benchmark's large case: d=6 n=8 (n^d = 262144) stride=67, nb=896, 5 outputs, "
+                  "N(0,1) data; multi-pass sweep through the workspace"),
 }
 
 

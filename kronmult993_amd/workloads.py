@@ -241,6 +241,12 @@ def random_case(seed: int, d: int, n: int, nb: int, stride: Optional[int] = None
     return hb.astype(dtype)
 
 
+def ref_large(seed: int = 0) -> HostBatch:
+    """The reference benchmark's "large" case (tests/kronmult_bench_gpu.cpp: degree 8, dimension 6, level 7):
+    896 entries of d=6, n=8 (n^d = 262144), stride 67, 5 distinct outputs."""
+    return reference_case(8, 6, 7, seed=seed)
+
+
 def by_name(name: str, seed: int = 0):
-    table = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}
+    table = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5, "ref_large": ref_large}
     return table[name](seed=seed)
