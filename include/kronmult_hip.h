@@ -53,7 +53,8 @@ int kronmult_batched_f32_async(int matrix_count, int matrix_size, float const *c
  * Launch-shape control for the stream-ordered entry points (no counterpart in the reference, which has a
  * single launch shape, kronmult_gpu/kronmult.cu:184-193). Zero fields mean "library default".
  *   family:                 KRONMULT_FAMILY_AUTO, _REGISTER (register-resident sweep, only for the
- *                           (d, n) plans it is built for) or _FIBER (fiber sweep, any n <= 64);
+ *                           (d, n) plans it is built for), _FIBER (fiber sweep, any n <= 64) or _MATRIX
+ *                           (every contraction on the FP64 matrix cores; d = 6, n = 4, double only);
  *   workgroups_per_cu:      resident workgroups to launch per compute unit (register family);
  *   entries_per_workgroup:  consecutive entries one workgroup walks (register family).
  * An impossible request (a family that does not cover (d, n)) returns hipErrorInvalidValue.
@@ -61,6 +62,7 @@ int kronmult_batched_f32_async(int matrix_count, int matrix_size, float const *c
 #define KRONMULT_FAMILY_AUTO 0
 #define KRONMULT_FAMILY_REGISTER 1
 #define KRONMULT_FAMILY_FIBER 2
+#define KRONMULT_FAMILY_MATRIX 3
 
 typedef struct kronmult_tuning
 {
@@ -71,7 +73,7 @@ typedef struct kronmult_tuning
 
 typedef struct kronmult_launch_info
 {
-    int family;                /* KRONMULT_FAMILY_REGISTER or _FIBER */
+    int family;                /* KRONMULT_FAMILY_REGISTER, _FIBER or _MATRIX */
     int grid;                  /* workgroups launched */
     int block;                 /* threads per workgroup */
     int entries_per_workgroup; /* entries walked by one workgroup */

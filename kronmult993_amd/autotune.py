@@ -7,8 +7,8 @@ is process-wide, guarded by a lock, and optionally persisted as JSON (path argum
 variable KRONMULT_TUNE_CACHE).
 
 Choices:
-  * one batch: the kernel family (register-resident sweep where a plan exists, fiber sweep) and, for the
-    register family, workgroups per CU;
+  * one batch: the kernel family (register-resident sweep where a plan exists, matrix-core sweep for
+    d=6 n=4 FP64, fiber sweep) and, for the register and matrix-core families, workgroups per CU;
   * several batches (a grouped call): the persistent launch -- workgroups per CU and per-group tile length
     (entries per tile, from a quarter of the entries the launch processes in one pass, which also shrinks
     the pass and its LDS, to four passes) -- against one launch per group with each group's own best
@@ -123,6 +123,11 @@ class Autotuner:
         try:
             api.launch_info(db.d, db.n, db.itemsize, db.nb, {"family": api.FAMILY_REGISTER})
             cands += [{"family": api.FAMILY_REGISTER, "workgroups_per_cu": w} for w in (1, 2, 4, 8, 16)]
+        except api.KronmultError:
+            pass
+        try:
+            api.launch_info(db.d, db.n, db.itemsize, db.nb, {"family": api.FAMILY_MATRIX})
+            cands += [{"family": api.FAMILY_MATRIX, "workgroups_per_cu": w} for w in (2, 3, 4)]
         except api.KronmultError:
             pass
         best, best_t = None, float("inf")

@@ -25,6 +25,7 @@
 
 #include "kron_adjoint.hpp"
 #include "kron_group.hpp"
+#include "kron_mfma.hpp"
 #include "kron_pass.hpp"
 #include "kron_split.hpp"
 #include "kron_reg.hpp"
@@ -232,6 +233,30 @@ hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T 
     return hipGetLastError();
 }
 
+// ---- matrix-core family (d = 6, n = 4, FP64; opt-in) -----------------------------------------------------
+// Same run structure as the register family. Not the default: on MI355X the FP64 matrix rate equals the
+// vector rate, and this kernel measured 67 us against the register sweep's 53 us on the d=6 n=4 batch.
+hipError_t mfma_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, double const *const *mats,
+                       double const *const *in, double *const *out, hipStream_t stream, bool run)
+{
+    auto kernel = kronmfma::kron_mfma_d6n4_kernel;
+    L.family = KRONMULT_FAMILY_MATRIX;
+    L.block = 256;
+    L.lds = sizeof(double) * 4096;
+    L.resident = resident_per_cu(kernel, 256, 0);
+    int const per_cu = tu.workgroups_per_cu > 0 ? tu.workgroups_per_cu : 3;  // measured best: 3 per CU
+    long const slots = static_cast<long>(device_cu_count()) * per_cu;
+    int epw = tu.entries_per_workgroup > 0 ? tu.entries_per_workgroup : static_cast<int>((s.nb + slots - 1) / slots);
+    if (epw < 1) epw = 1;
+    L.epw = epw;
+    long const runs = (s.nb + epw - 1) / epw;
+    L.grid = (runs + 7) / 8 * 8;
+    if (!run) return hipSuccess;
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(L.grid)), dim3(256), 0, stream, s.nb, epw, mats, s.stride,
+                       in, out);
+    return hipGetLastError();
+}
+
 template<typename T>
 hipError_t reg_dispatch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T const *const *mats,
                         T const *const *in, T *const *out, hipStream_t stream, bool run)
@@ -433,11 +458,13 @@ hipError_t fiber_dispatch(Shape const &s, Launch &L, T const *const *mats, T con
 }
 
 // ---- entry --------------------------------------------------------------------------------------------
-int choose_family(Shape const &s, kronmult_tuning_t const &tu)
+int choose_family(Shape const &s, kronmult_tuning_t const &tu, bool f64)
 {
     bool const reg_ok = reg_rho(s.d, s.n) > 0;
+    bool const mfma_ok = f64 && s.d == 6 && s.n == 4;
     if (tu.family == KRONMULT_FAMILY_REGISTER) return reg_ok ? KRONMULT_FAMILY_REGISTER : -1;
     if (tu.family == KRONMULT_FAMILY_FIBER) return s.n <= kMaxFiber ? KRONMULT_FAMILY_FIBER : -1;
+    if (tu.family == KRONMULT_FAMILY_MATRIX) return mfma_ok ? KRONMULT_FAMILY_MATRIX : -1;
     if (tu.family != KRONMULT_FAMILY_AUTO) return -1;
     if (reg_ok) return KRONMULT_FAMILY_REGISTER;
     return s.n <= kMaxFiber ? KRONMULT_FAMILY_FIBER : -1;
@@ -452,7 +479,7 @@ hipError_t kronmult_launch(int d, int n, T const *const *mats, int stride, T *co
     if (err != hipSuccess) return err;
     kronmult_tuning_t const tu = tuning ? *tuning : kronmult_tuning_t{0, 0, 0};
     if (tu.workgroups_per_cu < 0 || tu.entries_per_workgroup < 0) return hipErrorInvalidValue;
-    int const family = choose_family(s, tu);
+    int const family = choose_family(s, tu, sizeof(T) == 8);
     if (family < 0) return hipErrorInvalidValue;
     bool const run = info == nullptr;
     if (run)
@@ -464,6 +491,8 @@ hipError_t kronmult_launch(int d, int n, T const *const *mats, int stride, T *co
     Launch &L = info ? *info : local;
     T const *const *cin = reinterpret_cast<T const *const *>(in);
     if (family == KRONMULT_FAMILY_REGISTER) return reg_dispatch<T>(s, tu, L, mats, cin, out, stream, run);
+    if constexpr (sizeof(T) == 8)
+        if (family == KRONMULT_FAMILY_MATRIX) return mfma_launch(s, tu, L, mats, cin, out, stream, run);
     return fiber_dispatch<T>(s, L, mats, cin, out, ws, stream, run);
 }
 
@@ -484,8 +513,9 @@ char const *kernel_name(int d, int n, int nb)
     Shape s{};
     if (validate(d, n, n, nb < 0 ? 0 : nb, s) != hipSuccess) return "invalid";
     bool const f64 = sizeof(T) == 8;
-    switch (choose_family(s, kronmult_tuning_t{0, 0, 0}))
+    switch (choose_family(s, kronmult_tuning_t{0, 0, 0}, f64))
     {
+    case KRONMULT_FAMILY_MATRIX: return "mfma_sweep_f64";
     case KRONMULT_FAMILY_REGISTER: return f64 ? "reg_sweep_f64" : "reg_sweep_f32";
     case KRONMULT_FAMILY_FIBER:
         if (fiber_in_lds<T>(s)) return f64 ? "fiber_lds_f64" : "fiber_lds_f32";

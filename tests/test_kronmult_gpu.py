@@ -190,3 +190,41 @@ def test_async_on_a_stream_and_repeat_accumulates(gpu):
     once = oracle.kronmult(b)
     twice = oracle.kronmult(b, out_pool=once.copy())
     assert normwise(db.output(), twice) <= 1e-13
+
+
+def _run_tuned(b, gpu, tuning):
+    print(f"[case] {b.name} d={b.d} n={b.n} nb={b.nb} stride={b.stride} dtype={b.dtype} tuning={tuning}", flush=True)
+    db = DeviceBatch(b, device=str(gpu))
+    db.run(tuning=tuning)
+    return db.output()
+
+
+@pytest.mark.parametrize("epw", [0, 1, 3])
+def test_matrix_core_family_d6n4(gpu, epw):
+    """The opt-in matrix-core sweep (d=6, n=4, FP64): per entry bit-identical to the oracle, aliased outputs
+    within the bound, also with a padded stride, element-aligned inputs and short entry runs."""
+    tu = {"family": api.FAMILY_MATRIX, "entries_per_workgroup": epw}
+    u = _unaliased(W.c2(rows=3, cols=64))
+    _check_bitexact(u, _run_tuned(u, gpu, tu))
+    b = W.c2()
+    _check_aliased(b, _run_tuned(b, gpu, tu))
+    r = W.random_case(77, 6, 4, 97, stride=7, shared_factors=13)
+    size = r.size
+    pool = np.zeros(r.nb * (size + 1))
+    in_off = np.arange(r.nb, dtype=np.int64) * (size + 1) + (np.arange(r.nb) % 2)
+    for k in range(r.nb):
+        pool[in_off[k]: in_off[k] + size] = r.in_pool[r.in_off[k]: r.in_off[k] + size]
+    r = W.HostBatch(**{**r.__dict__, "in_pool": pool, "in_off": in_off})
+    _check_aliased(r, _run_tuned(r, gpu, tu))
+    ru = _unaliased(r)
+    _check_bitexact(ru, _run_tuned(ru, gpu, tu))
+
+
+def test_matrix_core_family_rejects_other_shapes(gpu):
+    tu = {"family": api.FAMILY_MATRIX}
+    assert api.launch_info(6, 4, 8, 100, tu)["family"] == api.FAMILY_MATRIX
+    for d, n, size in ((6, 4, 4), (5, 4, 8), (6, 8, 8), (3, 4, 8)):
+        with pytest.raises(api.KronmultError) as e:
+            api.launch_info(d, n, size, 100, tu)
+        assert e.value.code == api.HIP_ERROR_INVALID_VALUE
+    assert api.kernel_name(6, 4, 8, 100) == "reg_sweep_f64"
