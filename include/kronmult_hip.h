@@ -99,6 +99,40 @@ int kronmult_hip_launch_info(int matrix_count, int matrix_size, int element_size
                              kronmult_tuning_t const *tuning, kronmult_launch_info_t *info);
 
 /*
+ * Ordered accumulation (no counterpart in the reference, whose epilogue adds every element of every entry
+ * with an atomicAdd, kronmult_gpu/kronmult.cu:124-129, so aliased outputs change in their last bits from
+ * run to run). Same parameters and meaning as kronmult_batched_f64/_f32, plus:
+ *   - entries that share an output pointer must be consecutive in the batch (sorted or grouped by output),
+ *     and distinct output vectors must not overlap;
+ *   - workspace_batched[k] (n^d elements, disjoint from every other vector) is required for every entry:
+ *     it receives partial sums.
+ * The sum into each output is taken in a fixed order: per range of `entries_per_workgroup` entries
+ * (kronmult_hip_ordered_launch_info; 1 for the one-entry-per-workgroup kernels) the entries of one output
+ * are summed in entry order, P = ((0 + y_a) + y_a+1) + ...; then out = out + P if that is the output's only
+ * range, else S = P_0, S = S + P_1, ... in range order and out = out + S. Results are bitwise
+ * reproducible for a given batch and launch shape. The synchronous entry points check the layout on the
+ * host (copies of the output and workspace pointer arrays) and return hipErrorInvalidValue if it does not
+ * hold; the _async ones trust the caller. Not available for KRONMULT_FAMILY_MATRIX.
+ */
+int kronmult_batched_ordered_f64(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                                 int matrix_stride, double **input_batched, double **output_batched,
+                                 double **workspace_batched, int nb_batch);
+int kronmult_batched_ordered_f32(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
+                                 int matrix_stride, float **input_batched, float **output_batched,
+                                 float **workspace_batched, int nb_batch);
+int kronmult_batched_ordered_f64_async(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                                       int matrix_stride, double **input_batched, double **output_batched,
+                                       double **workspace_batched, int nb_batch, kronmult_tuning_t const *tuning,
+                                       void *stream);
+int kronmult_batched_ordered_f32_async(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
+                                       int matrix_stride, float **input_batched, float **output_batched,
+                                       float **workspace_batched, int nb_batch, kronmult_tuning_t const *tuning,
+                                       void *stream);
+/* The launch of an ordered call; entries_per_workgroup is the range length of the summation order. */
+int kronmult_hip_ordered_launch_info(int matrix_count, int matrix_size, int element_size, int nb_batch,
+                                     kronmult_tuning_t const *tuning, kronmult_launch_info_t *info);
+
+/*
  * Adjoint (no counterpart in the reference, which has no transposed or gradient API). For the cotangent
  * g_o = grad_output_batched-addressed vectors on the outputs of the forward call above (same pointer
  * arrays, so entries that share an output read the same g), accumulates for every entry k:

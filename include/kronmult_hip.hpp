@@ -56,3 +56,37 @@ inline hipError_t kronmult_batched<float>(int const matrix_count, int const matr
                                                         matrix_stride, input_batched, output_batched,
                                                         workspace_batched, nb_batch));
 }
+
+/*
+ * Same computation with a fixed summation order into shared outputs (bitwise reproducible; no counterpart
+ * in the reference). Entries of one output must be consecutive and every entry needs its workspace; the
+ * layout is checked and hipErrorInvalidValue returned otherwise (kronmult_batched_ordered_f64 in
+ * kronmult_hip.h).
+ */
+template<typename T>
+hipError_t kronmult_batched_ordered(int const matrix_count, int const matrix_size,
+                                    T const *const matrix_list_batched[], int const matrix_stride,
+                                    T *input_batched[], T *output_batched[], T *workspace_batched[],
+                                    int const nb_batch);
+
+template<>
+inline hipError_t kronmult_batched_ordered<double>(int const matrix_count, int const matrix_size,
+                                                   double const *const matrix_list_batched[], int const matrix_stride,
+                                                   double *input_batched[], double *output_batched[],
+                                                   double *workspace_batched[], int const nb_batch)
+{
+    return static_cast<hipError_t>(kronmult_batched_ordered_f64(matrix_count, matrix_size, matrix_list_batched,
+                                                                matrix_stride, input_batched, output_batched,
+                                                                workspace_batched, nb_batch));
+}
+
+template<>
+inline hipError_t kronmult_batched_ordered<float>(int const matrix_count, int const matrix_size,
+                                                  float const *const matrix_list_batched[], int const matrix_stride,
+                                                  float *input_batched[], float *output_batched[],
+                                                  float *workspace_batched[], int const nb_batch)
+{
+    return static_cast<hipError_t>(kronmult_batched_ordered_f32(matrix_count, matrix_size, matrix_list_batched,
+                                                                matrix_stride, input_batched, output_batched,
+                                                                workspace_batched, nb_batch));
+}

@@ -118,6 +118,41 @@ def launch_info(matrix_count: int, matrix_size: int, element_size: int, nb_batch
     return info.as_dict()
 
 
+def kronmult_batched_ordered(matrix_count: int, matrix_size: int, matrix_list_batched, matrix_stride: int,
+                             input_batched, output_batched, workspace_batched, nb_batch: int, *, dtype,
+                             tuning=None, stream=None, sync: bool = True) -> None:
+    """Ordered accumulation (bitwise reproducible sums into shared outputs; entries of one output consecutive,
+    a workspace per entry). sync=True: the synchronous entry point, which checks the layout on the host;
+    sync=False: stream-ordered with an optional launch shape, layout trusted."""
+    sfx = _suffix(dtype)
+    if sync:
+        if tuning is not None:
+            raise ValueError("tuning needs sync=False")
+        rc = getattr(lib(), "kronmult_batched_ordered_" + sfx)(
+            matrix_count, matrix_size, _addr(matrix_list_batched), matrix_stride, _addr(input_batched),
+            _addr(output_batched), _addr(workspace_batched), nb_batch)
+    else:
+        tu = _tuning(tuning)
+        rc = getattr(lib(), "kronmult_batched_ordered_" + sfx + "_async")(
+            matrix_count, matrix_size, _addr(matrix_list_batched), matrix_stride, _addr(input_batched),
+            _addr(output_batched), _addr(workspace_batched), nb_batch, ctypes.byref(tu) if tu else None,
+            _stream_handle(stream))
+    if rc != HIP_SUCCESS:
+        raise KronmultError(rc, "kronmult_batched_ordered")
+
+
+def ordered_launch_info(matrix_count: int, matrix_size: int, element_size: int, nb_batch: int,
+                        tuning=None) -> dict:
+    """The launch of an ordered call (entries_per_workgroup: the range length of its summation order)."""
+    info = LaunchInfo()
+    tu = _tuning(tuning)
+    rc = lib().kronmult_hip_ordered_launch_info(matrix_count, matrix_size, element_size, nb_batch,
+                                                ctypes.byref(tu) if tu else None, ctypes.byref(info))
+    if rc != HIP_SUCCESS:
+        raise KronmultError(rc, "kronmult_hip_ordered_launch_info")
+    return info.as_dict()
+
+
 ADJOINT_AUTO, ADJOINT_RECOMPUTE, ADJOINT_STORE = 0, 1, 2
 _INTERMEDIATES = {"auto": ADJOINT_AUTO, "recompute": ADJOINT_RECOMPUTE, "store": ADJOINT_STORE}
 

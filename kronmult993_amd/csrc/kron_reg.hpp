@@ -313,6 +313,35 @@ __device__ __forceinline__ void flush(T *dst_generic, T (&acc)[PL::R], int t)
     }
 }
 
+// Ordered accumulation (kron_ordered.hpp): piece [ps, pe) into its output when it is the whole segment
+// (plain read-modify-write), else stored into ws[ps].
+template<typename T, typename PL>
+__device__ __forceinline__ void flush_piece(T *dst_generic, T (&acc)[PL::R], int t, T *const *out, T *const *ws,
+                                            int nb, int ps, int pe)
+{
+    bool const whole = (ps == 0 || out[ps - 1] != dst_generic) && (pe == nb || out[pe] != dst_generic);
+    if (whole)
+    {
+        global_ptr<T> dst = as_global(dst_generic);
+#pragma unroll
+        for (int j = 0; j < PL::R; ++j)
+        {
+            dst[j * PL::P + t] = dst[j * PL::P + t] + acc[j];
+            acc[j] = T(0);
+        }
+    }
+    else
+    {
+        global_ptr<T> dst = as_global(ws[ps]);
+#pragma unroll
+        for (int j = 0; j < PL::R; ++j)
+        {
+            dst[j * PL::P + t] = acc[j];
+            acc[j] = T(0);
+        }
+    }
+}
+
 // Workgroup indices are dealt to the 8 XCDs round-robin (XCD = blockIdx % 8). Workgroup b takes the entry
 // run of logical index (b % 8) * (grid / 8) + b / 8, so each XCD walks one contiguous stretch of the batch:
 // entries that share inputs and outputs (consecutive in the callers' orderings) stay on one XCD's L2.
@@ -331,10 +360,11 @@ constexpr int min_waves_per_simd()
     return (PL::kD == 6 && PL::kN == 4 && sizeof(T) == 8) ? 3 : (PL::kD == 5 && PL::kN == 4 && sizeof(T) == 8) ? 3 : 1;
 }
 
-template<typename T, typename PL>
+// ORDERED: the accumulator is flushed per piece (flush_piece) instead of with atomics; ws is read only then.
+template<typename T, typename PL, bool ORDERED>
 __global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(min_waves_per_simd<T, PL>())))
     kron_reg_kernel(int nb, int epw, T const *const *__restrict__ mats, int stride, T const *const *__restrict__ in,
-                    T *const *__restrict__ out)
+                    T *const *__restrict__ out, T *const *__restrict__ ws)
 {
     static_assert(PL::NPHASE >= 1, "");
     constexpr bool kCoefLds = coef_in_lds<PL>();
@@ -355,6 +385,7 @@ __global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(min_
     load_input<T, PL>(in[e0], pre, t);
     if constexpr (kCoefLds) Coef<T, PL>::load(mats + static_cast<size_t>(e0) * PL::kD, stride, pre_coef, t);
     T *cur = out[e0];
+    int ps = e0;  // first entry of the current piece (entries with output cur)
 
     for (int e = e0; e < e1; ++e)
     {
@@ -374,13 +405,20 @@ __global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(min_
         T *dst = out[e];
         if (dst != cur)
         {
-            flush<T, PL>(cur, acc, t);
+            if constexpr (ORDERED)
+                flush_piece<T, PL>(cur, acc, t, out, ws, nb, ps, e);
+            else
+                flush<T, PL>(cur, acc, t);
             cur = dst;
+            ps = e;
         }
 #pragma unroll
         for (int j = 0; j < PL::R; ++j) acc[j] += v[j];
     }
-    flush<T, PL>(cur, acc, t);
+    if constexpr (ORDERED)
+        flush_piece<T, PL>(cur, acc, t, out, ws, nb, ps, e1);
+    else
+        flush<T, PL>(cur, acc, t);
 }
 
 } // namespace kronreg

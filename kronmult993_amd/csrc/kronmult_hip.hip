@@ -18,14 +18,17 @@
 //     hardware FP64/FP32 atomics (global_atomic_add_f64/f32).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <climits>
 #include <cstdint>
 #include <cstdio>
+#include <vector>
 
 #include "kron_adjoint.hpp"
 #include "kron_group.hpp"
 #include "kron_mfma.hpp"
+#include "kron_ordered.hpp"
 #include "kron_pass.hpp"
 #include "kron_split.hpp"
 #include "kron_reg.hpp"
@@ -49,10 +52,12 @@ constexpr int kFiberThreads = 256;      // workgroup size of the fiber sweep
 // N > 0: compile-time fiber length, coefficients held in registers (uniform across the workgroup, so the
 // compiler loads them with scalar loads). N == 0: runtime n <= kMaxFiber, coefficients staged in LDS.
 // InLds: the vector lives in LDS; otherwise in ws[b] (the first step reads the input directly).
+// ORDERED: each entry is a piece (kron_ordered.hpp): added to its output when it is the output's only
+// entry, else left in ws[b] for the reduction.
 // ------------------------------------------------------------------------------------------------------
-template<typename T, int N, bool InLds>
+template<typename T, int N, bool InLds, bool ORDERED>
 __global__ void __launch_bounds__(kFiberThreads)
-    kron_fiber_kernel(int d, int n_rt, int size, T const *const *__restrict__ mats, int stride,
+    kron_fiber_kernel(int d, int n_rt, int size, int nb, T const *const *__restrict__ mats, int stride,
                       T const *const *__restrict__ in, T *const *__restrict__ out, T *const *__restrict__ ws)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -127,7 +132,22 @@ __global__ void __launch_bounds__(kFiberThreads)
     }
     __syncthreads();
     T *y = out[b];
-    for (int i = threadIdx.x; i < size; i += blockDim.x) atomicAdd(y + i, x[i]);
+    if constexpr (ORDERED)
+    {
+        if (kronordered::whole_segment(out, nb, b, b + 1, y))
+        {
+            for (int i = threadIdx.x; i < size; i += blockDim.x) y[i] = y[i] + x[i];
+        }
+        else if (InLds)
+        {
+            T *w = ws[b];
+            for (int i = threadIdx.x; i < size; i += blockDim.x) w[i] = x[i];
+        }
+    }
+    else
+    {
+        for (int i = threadIdx.x; i < size; i += blockDim.x) atomicAdd(y + i, x[i]);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -137,6 +157,7 @@ struct Shape
 {
     int d, n, stride, nb;
     long size;
+    bool ordered = false;  // ordered accumulation (kron_ordered.hpp) instead of atomics
 };
 
 hipError_t validate(int d, int n, int stride, int nb, Shape &s)
@@ -148,7 +169,7 @@ hipError_t validate(int d, int n, int stride, int nb, Shape &s)
         size *= n;
         if (size > INT_MAX) return hipErrorInvalidValue;
     }
-    s = Shape{d, n, stride, nb, size};
+    s = Shape{d, n, stride, nb, size, false};
     return hipSuccess;
 }
 
@@ -211,11 +232,11 @@ constexpr int default_workgroups_per_cu(int threads)
 }
 template<typename T, int D, int N>
 hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T const *const *mats,
-                      T const *const *in, T *const *out, hipStream_t stream, bool run)
+                      T const *const *in, T *const *out, T *const *ws, hipStream_t stream, bool run)
 {
     using PL = kronplan::Plan<D, N, reg_rho(D, N)>;
     static_assert(PL::P % 64 == 0 && PL::P <= 1024, "plan must fill whole wavefronts");
-    auto kernel = kronreg::kron_reg_kernel<T, PL>;
+    auto kernel = s.ordered ? kronreg::kron_reg_kernel<T, PL, true> : kronreg::kron_reg_kernel<T, PL, false>;
     L.family = KRONMULT_FAMILY_REGISTER;
     L.block = PL::P;
     L.lds = sizeof(T) * (PL::SIZE + (kronreg::coef_in_lds<PL>() ? kronreg::Coef<T, PL>::kCount : 0));
@@ -229,7 +250,7 @@ hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T 
     L.grid = (runs + 7) / 8 * 8;  // surplus runs exit at once
     if (!run) return hipSuccess;
     hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(L.grid)), dim3(PL::P), 0, stream, s.nb, epw, mats,
-                       s.stride, in, out);
+                       s.stride, in, out, ws);
     return hipGetLastError();
 }
 
@@ -259,13 +280,13 @@ hipError_t mfma_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, d
 
 template<typename T>
 hipError_t reg_dispatch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T const *const *mats,
-                        T const *const *in, T *const *out, hipStream_t stream, bool run)
+                        T const *const *in, T *const *out, T *const *ws, hipStream_t stream, bool run)
 {
-    if (s.d == 6 && s.n == 4) return reg_launch<T, 6, 4>(s, tu, L, mats, in, out, stream, run);
-    if (s.d == 5 && s.n == 4) return reg_launch<T, 5, 4>(s, tu, L, mats, in, out, stream, run);
-    if (s.d == 4 && s.n == 4) return reg_launch<T, 4, 4>(s, tu, L, mats, in, out, stream, run);
-    if (s.d == 4 && s.n == 8) return reg_launch<T, 4, 8>(s, tu, L, mats, in, out, stream, run);
-    if (s.d == 3 && s.n == 8) return reg_launch<T, 3, 8>(s, tu, L, mats, in, out, stream, run);
+    if (s.d == 6 && s.n == 4) return reg_launch<T, 6, 4>(s, tu, L, mats, in, out, ws, stream, run);
+    if (s.d == 5 && s.n == 4) return reg_launch<T, 5, 4>(s, tu, L, mats, in, out, ws, stream, run);
+    if (s.d == 4 && s.n == 4) return reg_launch<T, 4, 4>(s, tu, L, mats, in, out, ws, stream, run);
+    if (s.d == 4 && s.n == 8) return reg_launch<T, 4, 8>(s, tu, L, mats, in, out, ws, stream, run);
+    if (s.d == 3 && s.n == 8) return reg_launch<T, 3, 8>(s, tu, L, mats, in, out, ws, stream, run);
     return hipErrorInvalidValue;
 }
 
@@ -287,7 +308,7 @@ hipError_t fiber_launch(Shape const &s, Launch &L, T const *const *mats, T const
     size_t lds = 0;
     if (InLds) lds += static_cast<size_t>(s.size) * sizeof(T);
     if (N == 0) lds += static_cast<size_t>(s.n) * s.n * sizeof(T);
-    auto kernel = kron_fiber_kernel<T, N, InLds>;
+    auto kernel = s.ordered ? kron_fiber_kernel<T, N, InLds, true> : kron_fiber_kernel<T, N, InLds, false>;
     L.family = KRONMULT_FAMILY_FIBER;
     L.grid = s.nb;
     L.block = threads;
@@ -296,8 +317,8 @@ hipError_t fiber_launch(Shape const &s, Launch &L, T const *const *mats, T const
     L.workspace = !InLds;
     L.resident = resident_per_cu(kernel, threads, lds);
     if (!run) return hipSuccess;
-    hipLaunchKernelGGL(kernel, dim3(s.nb), dim3(threads), lds, stream, s.d, s.n, static_cast<int>(s.size), mats,
-                       s.stride, in, out, ws);
+    hipLaunchKernelGGL(kernel, dim3(s.nb), dim3(threads), lds, stream, s.d, s.n, static_cast<int>(s.size), s.nb,
+                       mats, s.stride, in, out, ws);
     return hipGetLastError();
 }
 
@@ -395,7 +416,7 @@ hipError_t pass_run(Shape const &s, Launch &L, T const *const *mats, T const *co
         P.tiles_i = static_cast<int>(inner / TI);
         P.tiles_per_entry = static_cast<int>((outer / TO) * (inner / TI));
         P.first = q == 0;
-        P.last = q == passes - 1;
+        P.last = q == passes - 1 && !s.ordered;  // ordered: the last pass also writes the workspace
         P.nb = s.nb;
         P.epw = P.last ? kPassLastRun : 1;
         long const grid = static_cast<long>((s.nb + P.epw - 1) / P.epw) * P.tiles_per_entry;
@@ -470,30 +491,59 @@ int choose_family(Shape const &s, kronmult_tuning_t const &tu, bool f64)
     return s.n <= kMaxFiber ? KRONMULT_FAMILY_FIBER : -1;
 }
 
+// The reduction of an ordered call (kron_ordered.hpp): ranges of L.epw entries (the pieces' length), every
+// segment reduced when the compute kernels left all contributions in the workspace (multi-pass).
+template<typename T>
+hipError_t reduce_pieces(Shape const &s, Launch const &L, T *const *out, T *const *ws, hipStream_t stream)
+{
+    using namespace kronordered;
+    bool const all = L.workspace && use_passes<T>(s);
+    int const epw = all ? 1 : L.epw;
+    long const ranges = (s.nb + epw - 1) / epw;
+    long const chunks = (s.size + kReduceThreads * kReducePer - 1) / (kReduceThreads * kReducePer);
+    long const gy = chunks < 65535 ? chunks : 65535;
+    // about 1024 workgroups: ranges per workgroup from 1 to 256 (one per thread)
+    long rpw = (ranges * gy + 1023) / 1024;
+    rpw = rpw < 1 ? 1 : rpw > kReduceThreads ? kReduceThreads : rpw;
+    long const gx = (ranges + rpw - 1) / rpw;
+    hipLaunchKernelGGL(kron_reduce_pieces_kernel<T>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)),
+                       dim3(kReduceThreads), 0, stream, s.nb, epw, static_cast<int>(s.size), all ? 1 : 0,
+                       static_cast<int>(rpw), out, reinterpret_cast<T const *const *>(ws));
+    return hipGetLastError();
+}
+
 template<typename T>
 hipError_t kronmult_launch(int d, int n, T const *const *mats, int stride, T *const *in, T *const *out,
-                           T *const *ws, int nb, kronmult_tuning_t const *tuning, hipStream_t stream, Launch *info)
+                           T *const *ws, int nb, kronmult_tuning_t const *tuning, hipStream_t stream, Launch *info,
+                           bool ordered = false)
 {
     Shape s{};
     hipError_t err = validate(d, n, stride, nb, s);
     if (err != hipSuccess) return err;
+    s.ordered = ordered;
     kronmult_tuning_t const tu = tuning ? *tuning : kronmult_tuning_t{0, 0, 0};
     if (tu.workgroups_per_cu < 0 || tu.entries_per_workgroup < 0) return hipErrorInvalidValue;
     int const family = choose_family(s, tu, sizeof(T) == 8);
-    if (family < 0) return hipErrorInvalidValue;
+    if (family < 0 || (ordered && family == KRONMULT_FAMILY_MATRIX)) return hipErrorInvalidValue;
     bool const run = info == nullptr;
     if (run)
     {
         if (nb == 0) return hipSuccess;
         if (mats == nullptr || in == nullptr || out == nullptr) return hipErrorInvalidValue;
+        if (ordered && ws == nullptr) return hipErrorInvalidValue;
     }
     Launch local;
     Launch &L = info ? *info : local;
     T const *const *cin = reinterpret_cast<T const *const *>(in);
-    if (family == KRONMULT_FAMILY_REGISTER) return reg_dispatch<T>(s, tu, L, mats, cin, out, stream, run);
-    if constexpr (sizeof(T) == 8)
-        if (family == KRONMULT_FAMILY_MATRIX) return mfma_launch(s, tu, L, mats, cin, out, stream, run);
-    return fiber_dispatch<T>(s, L, mats, cin, out, ws, stream, run);
+    if (family == KRONMULT_FAMILY_REGISTER)
+        err = reg_dispatch<T>(s, tu, L, mats, cin, out, ws, stream, run);
+    else if constexpr (sizeof(T) == 8)
+        err = family == KRONMULT_FAMILY_MATRIX ? mfma_launch(s, tu, L, mats, cin, out, stream, run)
+                                               : fiber_dispatch<T>(s, L, mats, cin, out, ws, stream, run);
+    else
+        err = fiber_dispatch<T>(s, L, mats, cin, out, ws, stream, run);
+    if (err != hipSuccess || !ordered) return err;
+    return run ? reduce_pieces<T>(s, L, out, ws, stream) : hipSuccess;
 }
 
 template<typename T>
@@ -889,6 +939,46 @@ hipError_t build_element_batch(int d, int n, int base, T const *const *coeff, in
     return hipGetLastError();
 }
 
+// Precondition of the ordered entry points, checked on the host from copies of the pointer arrays:
+// entries with one output are consecutive, every output and workspace pointer is non-null, and the output
+// vectors of distinct segments and all workspace vectors are pairwise disjoint.
+template<typename T>
+hipError_t check_ordered_batch(T *const *out, T *const *ws, int nb, long size)
+{
+    if (nb == 0) return hipSuccess;
+    if (out == nullptr || ws == nullptr) return hipErrorInvalidValue;
+    std::vector<T *> o(static_cast<size_t>(nb)), w(static_cast<size_t>(nb));
+    hipError_t err = hipMemcpy(o.data(), out, sizeof(T *) * nb, hipMemcpyDefault);
+    if (err == hipSuccess) err = hipMemcpy(w.data(), ws, sizeof(T *) * nb, hipMemcpyDefault);
+    if (err != hipSuccess) return err;
+    std::vector<uintptr_t> starts;
+    starts.reserve(static_cast<size_t>(nb) * 2);
+    for (int k = 0; k < nb; ++k)
+    {
+        if (o[k] == nullptr || w[k] == nullptr) return hipErrorInvalidValue;
+        if (k == 0 || o[k] != o[k - 1]) starts.push_back(reinterpret_cast<uintptr_t>(o[k]));
+        starts.push_back(reinterpret_cast<uintptr_t>(w[k]));
+    }
+    std::sort(starts.begin(), starts.end());
+    uintptr_t const bytes = static_cast<uintptr_t>(size) * sizeof(T);
+    for (size_t i = 0; i + 1 < starts.size(); ++i)
+        if (starts[i] + bytes > starts[i + 1]) return hipErrorInvalidValue;
+    return hipSuccess;
+}
+
+template<typename T>
+int kronmult_ordered_sync(int d, int n, T const *const *mats, int stride, T **in, T **out, T **ws, int nb)
+{
+    Shape s{};
+    hipError_t err = validate(d, n, stride, nb, s);
+    if (err != hipSuccess || nb == 0) return static_cast<int>(err);
+    err = check_ordered_batch<T>(out, ws, nb, s.size);
+    if (err != hipSuccess) return static_cast<int>(err);
+    err = kronmult_launch<T>(d, n, mats, stride, in, out, ws, nb, nullptr, nullptr, nullptr, true);
+    if (err != hipSuccess) return static_cast<int>(err);
+    return static_cast<int>(hipDeviceSynchronize());
+}
+
 } // namespace
 
 extern "C"
@@ -946,6 +1036,62 @@ int kronmult_batched_f32_tuned(int matrix_count, int matrix_size, float const *c
     return static_cast<int>(kronmult_launch<float>(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
                                                    input_batched, output_batched, workspace_batched, nb_batch,
                                                    tuning, static_cast<hipStream_t>(stream), nullptr));
+}
+
+int kronmult_batched_ordered_f64(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                                 int matrix_stride, double **input_batched, double **output_batched,
+                                 double **workspace_batched, int nb_batch)
+{
+    return kronmult_ordered_sync<double>(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
+                                         input_batched, output_batched, workspace_batched, nb_batch);
+}
+
+int kronmult_batched_ordered_f32(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
+                                 int matrix_stride, float **input_batched, float **output_batched,
+                                 float **workspace_batched, int nb_batch)
+{
+    return kronmult_ordered_sync<float>(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
+                                        input_batched, output_batched, workspace_batched, nb_batch);
+}
+
+int kronmult_batched_ordered_f64_async(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                                       int matrix_stride, double **input_batched, double **output_batched,
+                                       double **workspace_batched, int nb_batch, kronmult_tuning_t const *tuning,
+                                       void *stream)
+{
+    return static_cast<int>(kronmult_launch<double>(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
+                                                    input_batched, output_batched, workspace_batched, nb_batch,
+                                                    tuning, static_cast<hipStream_t>(stream), nullptr, true));
+}
+
+int kronmult_batched_ordered_f32_async(int matrix_count, int matrix_size, float const *const *matrix_list_batched,
+                                       int matrix_stride, float **input_batched, float **output_batched,
+                                       float **workspace_batched, int nb_batch, kronmult_tuning_t const *tuning,
+                                       void *stream)
+{
+    return static_cast<int>(kronmult_launch<float>(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
+                                                   input_batched, output_batched, workspace_batched, nb_batch,
+                                                   tuning, static_cast<hipStream_t>(stream), nullptr, true));
+}
+
+int kronmult_hip_ordered_launch_info(int matrix_count, int matrix_size, int element_size, int nb_batch,
+                                     kronmult_tuning_t const *tuning, kronmult_launch_info_t *info)
+{
+    if (info == nullptr) return static_cast<int>(hipErrorInvalidValue);
+    Launch L;
+    hipError_t err;
+    if (element_size == 8)
+        err = kronmult_launch<double>(matrix_count, matrix_size, nullptr, matrix_size, nullptr, nullptr, nullptr,
+                                      nb_batch, tuning, nullptr, &L, true);
+    else if (element_size == 4)
+        err = kronmult_launch<float>(matrix_count, matrix_size, nullptr, matrix_size, nullptr, nullptr, nullptr,
+                                     nb_batch, tuning, nullptr, &L, true);
+    else
+        err = hipErrorInvalidValue;
+    if (err != hipSuccess) return static_cast<int>(err);
+    *info = kronmult_launch_info_t{L.family, static_cast<int>(L.grid), L.block, L.epw, static_cast<int>(L.lds),
+                                   L.resident, 1};
+    return static_cast<int>(hipSuccess);
 }
 
 int kronmult_hip_launch_info(int matrix_count, int matrix_size, int element_size, int nb_batch,

@@ -73,6 +73,16 @@ class DeviceBatch:
         else:
             api.kronmult_batched_async(*args, dtype=self.dtype_name, stream=stream)
 
+    def run_ordered(self, sync: bool = True, stream=None, tuning=None) -> None:
+        """The ordered-accumulation call (needs per-entry workspaces: workspace="per_entry")."""
+        if self.ws_ptrs is None:
+            raise ValueError("the ordered call needs per-entry workspaces (workspace='per_entry')")
+        api.kronmult_batched_ordered(self.d, self.n, self.mat_ptrs, self.stride, self.in_ptrs, self.out_ptrs,
+                                     self.ws_ptrs, self.nb, dtype=self.dtype_name, tuning=tuning, stream=stream,
+                                     sync=sync and tuning is None)
+        if sync and tuning is not None:
+            _torch().cuda.synchronize(self.device)
+
     # ---- adjoint -----------------------------------------------------------------------------------------
     def adjoint_setup(self, grad_out: Optional[np.ndarray] = None, want_input: bool = True,
                       want_mats: bool = True, intermediates="auto") -> None:

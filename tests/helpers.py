@@ -52,3 +52,46 @@ def accumulation_bound(b, ulp: float) -> np.ndarray:
     counts = np.bincount(b.out_index, minlength=b.out_count).astype(np.float64)
     per_elem = np.repeat(counts, b.size)
     return (per_elem + 2.0) * ulp * mag
+
+
+def grouped_by_output(b):
+    """The same entries reordered (stably) so that entries sharing an output are consecutive."""
+    from kronmult993_amd.workloads import HostBatch
+    p = np.argsort(b.out_index, kind="stable")
+    moff = b.mat_off.reshape(b.nb, b.d)[p].reshape(-1)
+    return HostBatch(**{**b.__dict__, "name": b.name + "_grouped", "mat_off": moff.copy(),
+                        "in_off": b.in_off[p].copy(), "out_index": b.out_index[p].copy()})
+
+
+def ordered_reference(b, epw: int) -> np.ndarray:
+    """The summation order of the ordered-accumulation call, from the oracle's per-entry contributions
+    (bit-identical to the device's): per output segment, pieces = its parts in ranges of `epw` entries,
+    P = ((0 + y_a) + y_a+1) + ...; out + P for a one-piece segment, else S = P_0 + P_1 + ... (left to
+    right) and out + S. Arithmetic in the batch's own precision."""
+    import oracle
+    from kronmult993_amd.workloads import HostBatch
+    u = HostBatch(**{**b.__dict__, "out_count": b.nb, "out_index": np.arange(b.nb, dtype=np.int64),
+                     "out_init": None})
+    y = oracle.kronmult(u).reshape(b.nb, b.size)
+    out = b.output_pool_init().reshape(b.out_count, b.size)
+    k = 0
+    while k < b.nb:
+        o = int(b.out_index[k])
+        s1 = k
+        while s1 < b.nb and int(b.out_index[s1]) == o:
+            s1 += 1
+        pieces = []
+        ps = k
+        while ps < s1:
+            pe = min(s1, (ps // epw + 1) * epw)
+            acc = np.zeros(b.size, dtype=b.dtype)
+            for e in range(ps, pe):
+                acc = acc + y[e]
+            pieces.append(acc)
+            ps = pe
+        total = pieces[0]
+        for p in pieces[1:]:
+            total = total + p
+        out[o] = out[o] + total
+        k = s1
+    return out.reshape(-1)

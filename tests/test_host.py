@@ -180,3 +180,21 @@ def test_builder_rejects_invalid_arguments_before_any_device_call():
     assert lib.kronmult_build_element_batch_f64(*args(6, 4, 0, 64, 2, 2)) == 1
     assert lib.kronmult_build_element_batch_f32(*args(6, 4, 16, 64, -1, 2)) == 1
     assert lib.kronmult_build_element_batch_f64(*args(6, 4, 16, 64, 0, 5)) == 0  # empty batch
+
+
+def test_ordered_reference_order_and_grouping():
+    """The ordered-sum model used by the ordered-accumulation tests: regrouping entries by output and any
+    range length give the oracle's result up to rounding; with one range per entry on integer-valued data
+    it is exact."""
+    import oracle
+    from helpers import grouped_by_output, normwise, ordered_reference
+    b = W.random_case(71, 3, 4, 50, distinct=6)
+    g = grouped_by_output(b)
+    assert np.all(np.diff(g.out_index) >= 0)
+    ref = oracle.kronmult(b)
+    assert normwise(oracle.kronmult(g), ref) <= 1e-14
+    for epw in (1, 3, 7, 50):
+        assert normwise(ordered_reference(g, epw), ref) <= 1e-14
+    ints = W.HostBatch(**{**g.__dict__, "mat_pool": np.round(g.mat_pool * 4), "in_pool": np.round(g.in_pool * 4),
+                          "out_init": np.round(g.out_init * 4)})
+    assert np.array_equal(ordered_reference(ints, 4), oracle.kronmult(ints))
