@@ -17,7 +17,7 @@ longer than the bytes that move at 8 TB/s, else HBM -- contract bytes per launch
 both rooflines (`roofline_compute`, `roofline_hbm`), the HBM traffic per launch measured with rocprofv3
 counters (profiles/traffic.json), and the reference's CPU implementation timed on this host.
 
-    python bench.py [--steps K] [--warmup W] [--config c2] [--dtype f64] [--op forward|adjoint|split]
+    python bench.py [--steps K] [--warmup W] [--config c2] [--dtype f64] [--op forward|adjoint|split|ordered]
                     [--intermediates auto|recompute|store] [--cpu-baseline auto|off] [--calibrate-copy]
 """
 import argparse
@@ -128,9 +128,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
-    ap.add_argument("--op", default="forward", choices=["forward", "adjoint", "split"],
+    ap.add_argument("--op", default="forward", choices=["forward", "adjoint", "split", "ordered"],
                     help="forward: kronmult_batched; adjoint: kronmult_batched_adjoint (input and factor "
-                         "gradients); split: FP64 data through the FP32 matrix cores (error-free splitting)")
+                         "gradients); split: FP64 data through the FP32 matrix cores (error-free splitting); "
+                         "ordered: kronmult_batched_ordered (reproducible sums into shared outputs)")
     ap.add_argument("--intermediates", default="auto", choices=["auto", "recompute", "store"],
                     help="adjoint: how the forward intermediates are obtained")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
@@ -155,7 +156,8 @@ def main() -> int:
     np_dtype = np.float64 if args.dtype == "f64" else np.float32
     batches = W.by_name(args.config)
     batches = [b.astype(np_dtype) for b in (batches if isinstance(batches, list) else [batches])]
-    dbs = [DeviceBatch(hb, device=str(dev)) for hb in batches]
+    dbs = [DeviceBatch(hb, device=str(dev), workspace="per_entry" if args.op == "ordered" else "auto")
+           for hb in batches]
     stream = torch.cuda.current_stream(dev)
     extra_config = {}
     if args.op != "forward" and len(dbs) != 1:
@@ -181,6 +183,9 @@ def main() -> int:
                 return 2
             kernel = "split_fixed3"
             step = lambda s: db.run_split("fixed3", sync=False, stream=s)
+        elif args.op == "ordered":
+            kernel = kernel + " (ordered) + reduce_pieces"
+            step = lambda s: db.run_ordered(sync=False, stream=s)
     else:
         from kronmult993_amd.autotune import Autotuner, grouped_call_from_choice
         choice = Autotuner().tune_grouped(dbs)
