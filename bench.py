@@ -17,7 +17,7 @@ longer than the bytes that move at 8 TB/s, else HBM -- contract bytes per launch
 both rooflines (`roofline_compute`, `roofline_hbm`), the HBM traffic per launch measured with rocprofv3
 counters (profiles/traffic.json), and the reference's CPU implementation timed on this host.
 
-    python bench.py [--steps K] [--warmup W] [--config c2] [--dtype f64] [--op forward|adjoint|split|ordered]
+    python bench.py [--steps K] [--warmup W] [--config c2] [--dtype f64] [--op forward|adjoint|split|ordered|terms]
                     [--intermediates auto|recompute|store] [--cpu-baseline auto|off] [--calibrate-copy]
 """
 import argparse
@@ -51,6 +51,8 @@ WORKLOADS = {
     "c5": "d=3 n=8 stride=8, nb=8192, 256 shared factor triples, 2048 outputs x 4 contributions",
     "ref_large": ("the reference benchmark's large case: d=6 n=8 (n^d = 262144) stride=67, nb=896, 5 outputs, "
                   "N(0,1) data; multi-pass sweep through the workspace"),
+    "c2_terms": ("the d=6 n=4 operator with 3 Kronecker terms per element pair (each term its own six 64x64 "
+                 "coefficient matrices), 8192 entries x 3 terms, 128 outputs; with --op terms"),
 }
 
 
@@ -128,10 +130,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
-    ap.add_argument("--op", default="forward", choices=["forward", "adjoint", "split", "ordered"],
+    ap.add_argument("--op", default="forward", choices=["forward", "adjoint", "split", "ordered", "terms"],
                     help="forward: kronmult_batched; adjoint: kronmult_batched_adjoint (input and factor "
                          "gradients); split: FP64 data through the FP32 matrix cores (error-free splitting); "
-                         "ordered: kronmult_batched_ordered (reproducible sums into shared outputs)")
+                         "ordered: kronmult_batched_ordered (reproducible sums into shared outputs); terms: "
+                         "kronmult_batched_terms on a several-term workload (--config c2_terms)")
     ap.add_argument("--intermediates", default="auto", choices=["auto", "recompute", "store"],
                     help="adjoint: how the forward intermediates are obtained")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
@@ -186,6 +189,15 @@ def main() -> int:
         elif args.op == "ordered":
             kernel = kernel + " (ordered) + reduce_pieces"
             step = lambda s: db.run_ordered(sync=False, stream=s)
+        elif args.op == "terms":
+            if "terms" not in hb.meta:
+                log("--op terms takes a several-term workload (--config c2_terms)")
+                return 2
+            base_nb = int(hb.meta["term_nb"])
+            fused = api.kernel_name(hb.d, hb.n, hb.dtype.itemsize, base_nb).startswith("reg_sweep")
+            kernel = ("reg_terms" if fused else "per-term launches") + f" ({hb.meta['terms']} terms)"
+            step = lambda s: db.run_terms(sync=False, stream=s)
+            extra_config.update({"terms": int(hb.meta["terms"]), "nb_batch": base_nb})
     else:
         from kronmult993_amd.autotune import Autotuner, grouped_call_from_choice
         choice = Autotuner().tune_grouped(dbs)

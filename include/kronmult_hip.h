@@ -128,6 +128,30 @@ int kronmult_batched_ordered_f32_async(int matrix_count, int matrix_size, float 
                                        int matrix_stride, float **input_batched, float **output_batched,
                                        float **workspace_batched, int nb_batch, kronmult_tuning_t const *tuning,
                                        void *stream);
+/*
+ * Several Kronecker terms per entry (no counterpart in the reference, which takes one term per call; the
+ * ASGarD operator is a sum of such terms over the same element pairs):
+ *   output_batched[k] += sum_{t < term_count} kron(A_{t,k,0}, ..., A_{t,k,d-1}) · input_batched[k],
+ * with the factor pointers term-major: A_{t,k,m} = matrix_list_batched[(t*nb_batch + k)*matrix_count + m]
+ * (term_count ordinary pointer lists, one after the other). Other parameters as kronmult_batched_f64.
+ * Where a register-resident plan exists the input is staged once per entry and the terms are summed in
+ * registers before one atomic flush; otherwise each term is one launch of the ordinary path.
+ */
+int kronmult_batched_terms_f64(int matrix_count, int matrix_size, int term_count,
+                               double const *const *matrix_list_batched, int matrix_stride, double **input_batched,
+                               double **output_batched, double **workspace_batched, int nb_batch);
+int kronmult_batched_terms_f32(int matrix_count, int matrix_size, int term_count,
+                               float const *const *matrix_list_batched, int matrix_stride, float **input_batched,
+                               float **output_batched, float **workspace_batched, int nb_batch);
+int kronmult_batched_terms_f64_async(int matrix_count, int matrix_size, int term_count,
+                                     double const *const *matrix_list_batched, int matrix_stride,
+                                     double **input_batched, double **output_batched, double **workspace_batched,
+                                     int nb_batch, void *stream);
+int kronmult_batched_terms_f32_async(int matrix_count, int matrix_size, int term_count,
+                                     float const *const *matrix_list_batched, int matrix_stride,
+                                     float **input_batched, float **output_batched, float **workspace_batched,
+                                     int nb_batch, void *stream);
+
 /* The launch of an ordered call; entries_per_workgroup is the range length of the summation order. */
 int kronmult_hip_ordered_launch_info(int matrix_count, int matrix_size, int element_size, int nb_batch,
                                      kronmult_tuning_t const *tuning, kronmult_launch_info_t *info);

@@ -90,3 +90,35 @@ inline hipError_t kronmult_batched_ordered<float>(int const matrix_count, int co
                                                                 matrix_stride, input_batched, output_batched,
                                                                 workspace_batched, nb_batch));
 }
+
+/*
+ * Sum of term_count Kronecker terms per entry (no counterpart in the reference): factor pointers term-major,
+ * term t of entry k at matrix_list_batched[(t*nb_batch + k)*matrix_count ...] (kronmult_batched_terms_f64
+ * in kronmult_hip.h).
+ */
+template<typename T>
+hipError_t kronmult_batched_terms(int const matrix_count, int const matrix_size, int const term_count,
+                                  T const *const matrix_list_batched[], int const matrix_stride, T *input_batched[],
+                                  T *output_batched[], T *workspace_batched[], int const nb_batch);
+
+template<>
+inline hipError_t kronmult_batched_terms<double>(int const matrix_count, int const matrix_size, int const term_count,
+                                                 double const *const matrix_list_batched[], int const matrix_stride,
+                                                 double *input_batched[], double *output_batched[],
+                                                 double *workspace_batched[], int const nb_batch)
+{
+    return static_cast<hipError_t>(kronmult_batched_terms_f64(matrix_count, matrix_size, term_count,
+                                                              matrix_list_batched, matrix_stride, input_batched,
+                                                              output_batched, workspace_batched, nb_batch));
+}
+
+template<>
+inline hipError_t kronmult_batched_terms<float>(int const matrix_count, int const matrix_size, int const term_count,
+                                                float const *const matrix_list_batched[], int const matrix_stride,
+                                                float *input_batched[], float *output_batched[],
+                                                float *workspace_batched[], int const nb_batch)
+{
+    return static_cast<hipError_t>(kronmult_batched_terms_f32(matrix_count, matrix_size, term_count,
+                                                              matrix_list_batched, matrix_stride, input_batched,
+                                                              output_batched, workspace_batched, nb_batch));
+}

@@ -153,6 +153,22 @@ def ordered_launch_info(matrix_count: int, matrix_size: int, element_size: int, 
     return info.as_dict()
 
 
+def kronmult_batched_terms(matrix_count: int, matrix_size: int, term_count: int, matrix_list_batched,
+                           matrix_stride: int, input_batched, output_batched, workspace_batched, nb_batch: int, *,
+                           dtype, stream=None, sync: bool = True) -> None:
+    """output[k] += sum_t kron(A_{t,k,*}) input[k]; factor pointers term-major (term t of entry k at
+    (t*nb_batch + k)*matrix_count). Synchronous, or stream-ordered with sync=False."""
+    sfx = _suffix(dtype)
+    args = (matrix_count, matrix_size, term_count, _addr(matrix_list_batched), matrix_stride, _addr(input_batched),
+            _addr(output_batched), _addr(workspace_batched), nb_batch)
+    if sync:
+        rc = getattr(lib(), "kronmult_batched_terms_" + sfx)(*args)
+    else:
+        rc = getattr(lib(), "kronmult_batched_terms_" + sfx + "_async")(*args, _stream_handle(stream))
+    if rc != HIP_SUCCESS:
+        raise KronmultError(rc, "kronmult_batched_terms")
+
+
 ADJOINT_AUTO, ADJOINT_RECOMPUTE, ADJOINT_STORE = 0, 1, 2
 _INTERMEDIATES = {"auto": ADJOINT_AUTO, "recompute": ADJOINT_RECOMPUTE, "store": ADJOINT_STORE}
 

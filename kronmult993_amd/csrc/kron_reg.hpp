@@ -421,4 +421,79 @@ __global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(min_
         flush<T, PL>(cur, acc, t);
 }
 
+// Several Kronecker terms per entry: output[e] += sum_t kron(A_{t,e,0..d-1}) input[e], t = 0..nterm-1, with
+// the factor pointers term-major (term t of entry e at mats[(t*nb + e)*d ...]). The input goes through the
+// LDS staging once per entry and its phase-0 layout stays in registers (x0); each term sweeps a copy of it
+// and is added to the accumulator in term order, so every term's contribution is the single-term kernel's
+// bit for bit. Factors staged in LDS (n = 8 plans) are loaded per term.
+template<typename T, typename PL>
+__global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(2)))
+    kron_reg_terms_kernel(int nb, int nterm, int epw, T const *const *__restrict__ mats, int stride,
+                          T const *const *__restrict__ in, T *const *__restrict__ out)
+{
+    static_assert(PL::NPHASE >= 1, "");
+    constexpr bool kCoefLds = coef_in_lds<PL>();
+    constexpr int kCoefSlots = kCoefLds ? Coef<T, PL>::kCount : 0;
+    __shared__ __attribute__((aligned(16))) T lds[PL::SIZE + kCoefSlots];
+    T *coef = lds + PL::SIZE;
+    int const t = threadIdx.x;
+    int const e0 = xcd_run_index() * epw;
+    int const e1 = (e0 + epw < nb) ? e0 + epw : nb;
+    if (e0 >= e1) return;
+    auto factors = [&](int term, int e) { return mats + (static_cast<size_t>(term) * nb + e) * PL::kD; };
+
+    T acc[PL::R];
+#pragma unroll
+    for (int j = 0; j < PL::R; ++j) acc[j] = T(0);
+    T pre[PL::R];
+    T pre_coef[kCoefLds ? Coef<T, PL>::kPerThread : 1];
+    T x0[PL::R], v[PL::R];
+    load_input<T, PL>(in[e0], pre, t);
+    if constexpr (kCoefLds) Coef<T, PL>::load(factors(0, e0), stride, pre_coef, t);
+    T *cur = out[e0];
+
+    for (int e = e0; e < e1; ++e)
+    {
+        T *dst = out[e];
+        if (dst != cur)
+        {
+            flush<T, PL>(cur, acc, t);
+            cur = dst;
+        }
+        lds_barrier();  // LDS free: the previous term's last phase has been read by everyone
+        Lds<T, PL>::store_input(lds, pre, t);
+        if constexpr (kCoefLds) Coef<T, PL>::store(coef, pre_coef, t);
+        lds_barrier();
+        if (e + 1 < e1) load_input<T, PL>(in[e + 1], pre, t);
+        Lds<T, PL>::template read<0>(lds, x0, t);
+        for (int term = 0; term < nterm; ++term)
+        {
+            if constexpr (kCoefLds)
+            {
+                if (term + 1 < nterm)
+                    Coef<T, PL>::load(factors(term + 1, e), stride, pre_coef, t);
+                else if (e + 1 < e1)
+                    Coef<T, PL>::load(factors(0, e + 1), stride, pre_coef, t);
+            }
+#pragma unroll
+            for (int j = 0; j < PL::R; ++j) v[j] = x0[j];
+            run_phases<T, PL, 0>(lds, v, factors(term, e), stride, coef, t);
+#pragma unroll
+            for (int j = 0; j < PL::R; ++j) acc[j] += v[j];
+            if constexpr (kCoefLds)
+            {
+                if (term + 1 < nterm)
+                {
+                    lds_barrier();  // everyone is done with this term's factors and last phase
+                    Coef<T, PL>::store(coef, pre_coef, t);
+                    // the next term's first LDS access comes after a barrier inside run_phases, or, for a
+                    // one-phase plan, not at all; its factor reads need this one
+                    lds_barrier();
+                }
+            }
+        }
+    }
+    flush<T, PL>(cur, acc, t);
+}
+
 } // namespace kronreg

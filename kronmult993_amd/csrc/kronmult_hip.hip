@@ -546,6 +546,50 @@ hipError_t kronmult_launch(int d, int n, T const *const *mats, int stride, T *co
     return run ? reduce_pieces<T>(s, L, out, ws, stream) : hipSuccess;
 }
 
+// ---- several Kronecker terms per entry -----------------------------------------------------------------
+template<typename T, int D, int N>
+hipError_t reg_terms_launch(Shape const &s, int nterm, T const *const *mats, T const *const *in, T *const *out,
+                            hipStream_t stream)
+{
+    using PL = kronplan::Plan<D, N, reg_rho(D, N)>;
+    auto kernel = kronreg::kron_reg_terms_kernel<T, PL>;
+    long const slots = static_cast<long>(device_cu_count()) * default_workgroups_per_cu(PL::P);
+    int epw = static_cast<int>((s.nb + slots - 1) / slots);
+    if (epw < 1) epw = 1;
+    long const runs = (s.nb + epw - 1) / epw;
+    long const grid = (runs + 7) / 8 * 8;
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(grid)), dim3(PL::P), 0, stream, s.nb, nterm, epw, mats,
+                       s.stride, in, out);
+    return hipGetLastError();
+}
+
+// The fused kernel where a register-resident plan exists, else one launch of the one-term path per term
+// (term t's factor pointers start at mats + t*nb*d).
+template<typename T>
+hipError_t terms_launch(int d, int n, int nterm, T const *const *mats, int stride, T *const *in, T *const *out,
+                        T *const *ws, int nb, hipStream_t stream)
+{
+    Shape s{};
+    hipError_t err = validate(d, n, stride, nb, s);
+    if (err != hipSuccess) return err;
+    if (nterm < 1) return hipErrorInvalidValue;
+    if (nb == 0) return hipSuccess;
+    if (mats == nullptr || in == nullptr || out == nullptr) return hipErrorInvalidValue;
+    T const *const *cin = reinterpret_cast<T const *const *>(in);
+    if (s.d == 6 && s.n == 4) return reg_terms_launch<T, 6, 4>(s, nterm, mats, cin, out, stream);
+    if (s.d == 5 && s.n == 4) return reg_terms_launch<T, 5, 4>(s, nterm, mats, cin, out, stream);
+    if (s.d == 4 && s.n == 4) return reg_terms_launch<T, 4, 4>(s, nterm, mats, cin, out, stream);
+    if (s.d == 4 && s.n == 8) return reg_terms_launch<T, 4, 8>(s, nterm, mats, cin, out, stream);
+    if (s.d == 3 && s.n == 8) return reg_terms_launch<T, 3, 8>(s, nterm, mats, cin, out, stream);
+    for (int t = 0; t < nterm; ++t)
+    {
+        err = kronmult_launch<T>(d, n, mats + static_cast<size_t>(t) * nb * d, stride, in, out, ws, nb, nullptr,
+                                 stream, nullptr);
+        if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+}
+
 template<typename T>
 int kronmult_sync(int d, int n, T const *const *mats, int stride, T **in, T **out, T **ws, int nb)
 {
@@ -1072,6 +1116,44 @@ int kronmult_batched_ordered_f32_async(int matrix_count, int matrix_size, float 
     return static_cast<int>(kronmult_launch<float>(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
                                                    input_batched, output_batched, workspace_batched, nb_batch,
                                                    tuning, static_cast<hipStream_t>(stream), nullptr, true));
+}
+
+int kronmult_batched_terms_f64(int matrix_count, int matrix_size, int term_count,
+                               double const *const *matrix_list_batched, int matrix_stride, double **input_batched,
+                               double **output_batched, double **workspace_batched, int nb_batch)
+{
+    hipError_t err = terms_launch<double>(matrix_count, matrix_size, term_count, matrix_list_batched, matrix_stride,
+                                          input_batched, output_batched, workspace_batched, nb_batch, nullptr);
+    return static_cast<int>(err != hipSuccess ? err : hipDeviceSynchronize());
+}
+
+int kronmult_batched_terms_f32(int matrix_count, int matrix_size, int term_count,
+                               float const *const *matrix_list_batched, int matrix_stride, float **input_batched,
+                               float **output_batched, float **workspace_batched, int nb_batch)
+{
+    hipError_t err = terms_launch<float>(matrix_count, matrix_size, term_count, matrix_list_batched, matrix_stride,
+                                         input_batched, output_batched, workspace_batched, nb_batch, nullptr);
+    return static_cast<int>(err != hipSuccess ? err : hipDeviceSynchronize());
+}
+
+int kronmult_batched_terms_f64_async(int matrix_count, int matrix_size, int term_count,
+                                     double const *const *matrix_list_batched, int matrix_stride,
+                                     double **input_batched, double **output_batched, double **workspace_batched,
+                                     int nb_batch, void *stream)
+{
+    return static_cast<int>(terms_launch<double>(matrix_count, matrix_size, term_count, matrix_list_batched,
+                                                 matrix_stride, input_batched, output_batched, workspace_batched,
+                                                 nb_batch, static_cast<hipStream_t>(stream)));
+}
+
+int kronmult_batched_terms_f32_async(int matrix_count, int matrix_size, int term_count,
+                                     float const *const *matrix_list_batched, int matrix_stride,
+                                     float **input_batched, float **output_batched, float **workspace_batched,
+                                     int nb_batch, void *stream)
+{
+    return static_cast<int>(terms_launch<float>(matrix_count, matrix_size, term_count, matrix_list_batched,
+                                                matrix_stride, input_batched, output_batched, workspace_batched,
+                                                nb_batch, static_cast<hipStream_t>(stream)));
 }
 
 int kronmult_hip_ordered_launch_info(int matrix_count, int matrix_size, int element_size, int nb_batch,

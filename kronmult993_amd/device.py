@@ -83,6 +83,14 @@ class DeviceBatch:
         if sync and tuning is not None:
             _torch().cuda.synchronize(self.device)
 
+    def run_terms(self, sync: bool = True, stream=None) -> None:
+        """The several-term call for a batch in expanded form (workloads.expand_terms): all term-major factor
+        pointers, the first term_nb input and output pointers."""
+        terms, nb = int(self.hb.meta["terms"]), int(self.hb.meta["term_nb"])
+        api.kronmult_batched_terms(self.d, self.n, terms, self.mat_ptrs, self.stride, self.in_ptrs[:nb],
+                                   self.out_ptrs[:nb], None if self.ws_ptrs is None else self.ws_ptrs[:nb], nb,
+                                   dtype=self.dtype_name, stream=stream, sync=sync)
+
     # ---- adjoint -----------------------------------------------------------------------------------------
     def adjoint_setup(self, grad_out: Optional[np.ndarray] = None, want_input: bool = True,
                       want_mats: bool = True, intermediates="auto") -> None:

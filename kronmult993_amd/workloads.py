@@ -143,6 +143,27 @@ def c2(seed: int = 0, rows: int = 128, cols: int = 64) -> HostBatch:
                      meta={"rows": rows, "cols": cols})
 
 
+def expand_terms(b: HostBatch, factor_pools) -> HostBatch:
+    """A several-term batch in expanded form: entry t*nb + k is term t of entry k of `b` (same input and
+    output), its factors at b's offsets in factor_pools[t] (laid out like b.mat_pool). The first nb input
+    and output pointers with all nb*terms*d factor pointers are the term-major call's arguments."""
+    terms = len(factor_pools)
+    pool = np.concatenate([np.asarray(f, dtype=b.dtype) for f in factor_pools])
+    moff = np.concatenate([b.mat_off + t * len(b.mat_pool) for t in range(terms)])
+    return replace(b, name=f"{b.name}_x{terms}terms", mat_pool=pool, mat_off=moff,
+                   in_off=np.tile(b.in_off, terms), out_index=np.tile(b.out_index, terms),
+                   meta=dict(b.meta, terms=terms, term_nb=b.nb))
+
+
+def c2_terms(seed: int = 0, terms: int = 3, rows: int = 128, cols: int = 64) -> HostBatch:
+    """The d=6 n=4 operator with `terms` Kronecker terms over the same element pairs: term tau draws its own
+    six 64x64 coefficient matrices (term 0 is the one-term workload's)."""
+    b = c2(seed, rows, cols)
+    pools = [np.concatenate([gen.normal(seed, 10 + t + 16 * tau, 64 * 64) / 8.0 for t in range(6)])
+             for tau in range(terms)]
+    return expand_terms(b, pools)
+
+
 C3_GROUPS = ((2, 8, 20000), (3, 6, 6000), (4, 4, 3000), (5, 3, 1500), (6, 2, 800))
 
 
@@ -241,6 +262,14 @@ def random_case(seed: int, d: int, n: int, nb: int, stride: Optional[int] = None
     return hb.astype(dtype)
 
 
+def random_terms(seed: int, d: int, n: int, nb: int, terms: int, **kw) -> HostBatch:
+    """random_case with `terms` factor sets per entry (expanded form, see expand_terms)."""
+    b = random_case(seed, d, n, nb, **kw)
+    pools = [b.mat_pool] + [gen.uniform(seed, 31 + 100 * t, len(b.mat_pool)).astype(b.dtype)
+                            for t in range(1, terms)]
+    return expand_terms(b, pools)
+
+
 def ref_large(seed: int = 0) -> HostBatch:
     """The reference benchmark's "large" case (tests/kronmult_bench_gpu.cpp: degree 8, dimension 6, level 7):
     896 entries of d=6, n=8 (n^d = 262144), stride 67, 5 distinct outputs."""
@@ -248,5 +277,5 @@ def ref_large(seed: int = 0) -> HostBatch:
 
 
 def by_name(name: str, seed: int = 0):
-    table = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5, "ref_large": ref_large}
+    table = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5, "ref_large": ref_large, "c2_terms": c2_terms}
     return table[name](seed=seed)

@@ -95,3 +95,28 @@ def ordered_reference(b, epw: int) -> np.ndarray:
         out[o] = out[o] + total
         k = s1
     return out.reshape(-1)
+
+
+def terms_reference(b, fused: bool) -> np.ndarray:
+    """Several-term call on a batch in expanded form (workloads.expand_terms) with every base entry writing
+    its own output: from the oracle's per-entry contributions y_{t,k}, out_k + (((0 + y_0k) + y_1k) + ...)
+    for the fused kernel (terms summed in registers), ((out_k + y_0k) + y_1k) + ... for one launch per term."""
+    import oracle
+    from kronmult993_amd.workloads import HostBatch
+    terms, nb = int(b.meta["terms"]), int(b.meta["term_nb"])
+    assert np.array_equal(np.sort(b.out_index[:nb]), np.arange(nb)), "base entries must have distinct outputs"
+    u = HostBatch(**{**b.__dict__, "out_count": b.nb, "out_index": np.arange(b.nb, dtype=np.int64),
+                     "out_init": None})
+    y = oracle.kronmult(u).reshape(terms, nb, b.size)
+    out = b.output_pool_init().reshape(b.out_count, b.size)
+    for k in range(nb):
+        o = int(b.out_index[k])
+        if fused:
+            acc = np.zeros(b.size, dtype=b.dtype)
+            for t in range(terms):
+                acc = acc + y[t, k]
+            out[o] = out[o] + acc
+        else:
+            for t in range(terms):
+                out[o] = out[o] + y[t, k]
+    return out.reshape(-1)
