@@ -100,7 +100,13 @@ def cpu_baseline(hbs, runs: int):
         log(f"[cpu] run {r} {'(warm-up) ' if r == 0 else ''}{(t1 - t0) * 1e3:.1f} ms on {threads} threads")
     med = statistics.median(times)
     flops = sum(hb.flops() for hb in hbs)
-    placement = f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} OMP_PLACES={os.environ.get('OMP_PLACES')}"
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    placement = (f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} OMP_PLACES={os.environ.get('OMP_PLACES')} "
+                 f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')}, {avail} CPUs in this process's affinity "
+                 f"mask, {os.cpu_count()} online")
     names = "+".join(hb.name for hb in hbs)
     nb = sum(hb.nb for hb in hbs)
     return {"value": round(flops / med / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": kind,
