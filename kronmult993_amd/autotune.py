@@ -56,7 +56,9 @@ def key_single(arch_name: str, dtype: str, d: int, n: int, nb: int) -> str:
 def entries_per_pass(d: int, n: int, itemsize: int) -> int:
     """Entries the persistent launch processes at once for (d, n) (same rule as the library)."""
     nfib = n ** (d - 1)
-    per_entry = (n ** d + d * n * n) * itemsize
+    size = n ** d
+    padded = size + size // n if n in (4, 8) else size   # kron_group.hpp: padded rows, 16-byte factor pad
+    per_entry = (padded + d * n * n + 2) * itemsize
     e = max(1, GROUP_THREADS // nfib)
     while e > 1 and e * per_entry > LDS_BUDGET:
         e -= 1

@@ -45,29 +45,51 @@ struct GroupTable
     GroupDesc<T> g[kMaxGroups];
 };
 
+// LDS layout of a pass. For n = 4 and 8 every row of n consecutive vector elements is followed by one
+// element of padding, and every entry's d factors by 16 bytes: the fastest mode's fibers (n elements apart
+// across lanes) and the factors of different entries (read as broadcasts, one address per entry) then fall
+// into different 16-byte banks instead of 4 of them.
+template<int N>
+constexpr bool pad_rows()
+{
+    return N == 4 || N == 8;
+}
+__host__ __device__ constexpr int padded_size(int n, int size)
+{
+    return (n == 4 || n == 8) ? size + size / n : size;
+}
+__host__ __device__ constexpr int coef_pitch(int d, int n)
+{
+    return d * n * n + 2;
+}
+
 // process entries [e0, e0 + cnt) of group G (cnt <= G.E) with N = G.n fixed at compile time
 template<typename T, int N>
-__device__ __forceinline__ void pass(GroupDesc<T> const &G, int e0, int cnt, T *lds)
+__device__ __forceinline__ void pass(GroupDesc<T> const &G, int e0, int cnt, T *lds, T **out_slot)
 {
     int const d = G.d, size = G.size, nfib = G.nfib;
     int const nn = N * N;
-    T *const x = lds;                               // cnt vectors of `size`
-    T *const coef = lds + static_cast<size_t>(G.E) * size;  // cnt x d factors, row-major a[r][k]
+    int const cp = coef_pitch(d, N);
+    auto px = [](int i) { return pad_rows<N>() ? i + i / N : i; };  // element i of the entry-major vectors
+    T *const x = lds;                                                           // cnt padded vectors
+    T *const coef = lds + static_cast<size_t>(G.E) * padded_size(N, size);      // entry q at q * cp: a[m][r][k]
     int const tid = threadIdx.x;
 
     for (int i = tid; i < cnt * size; i += blockDim.x)
     {
         int const q = i / size;
-        int const e = i - q * size;
-        x[i] = kronreg::as_global(G.in[e0 + q])[e];
+        x[px(i)] = kronreg::as_global(G.in[e0 + q])[i - q * size];
     }
+    if (tid < cnt) out_slot[tid] = G.out[e0 + tid];  // the pass's output pointers, read once
     int const ncoef = cnt * d * nn;
     for (int i = tid; i < ncoef; i += blockDim.x)
     {
-        int const qm = i / nn;               // q * d + m
+        int const qm = i / nn;  // q * d + m
         int const rk = i - qm * nn;
         int const r = rk / N, k = rk - (rk / N) * N;
-        coef[i] = kronreg::as_global(G.mats[static_cast<size_t>(e0) * d + qm])[r + k * G.stride];
+        int const q = qm / d;
+        coef[q * cp + (qm - q * d) * nn + rk] =
+            kronreg::as_global(G.mats[static_cast<size_t>(e0) * d + qm])[r + k * G.stride];
     }
     __syncthreads();
 
@@ -80,17 +102,17 @@ __device__ __forceinline__ void pass(GroupDesc<T> const &G, int e0, int cnt, T *
             int const f = w - q * nfib;
             int const hi = f / st;
             int const base = q * size + hi * st * N + (f - hi * st);
-            T const *a = coef + (q * d + t) * nn;
+            T const *a = coef + q * cp + t * nn;
             T v[N];
 #pragma unroll
-            for (int k = 0; k < N; ++k) v[k] = x[base + k * st];
+            for (int k = 0; k < N; ++k) v[k] = x[px(base + k * st)];
 #pragma unroll
             for (int r = 0; r < N; ++r)
             {
                 T acc = T(0);
 #pragma unroll
                 for (int k = 0; k < N; ++k) acc = fma(v[k], a[r * N + k], acc);
-                x[base + r * st] = acc;
+                x[px(base + r * st)] = acc;
             }
         }
         __syncthreads();
@@ -102,10 +124,10 @@ __device__ __forceinline__ void pass(GroupDesc<T> const &G, int e0, int cnt, T *
     {
         int const q = i / size;
         int const e = i - q * size;
-        T *const dst = G.out[e0 + q];
-        if (q > 0 && G.out[e0 + q - 1] == dst) continue;
-        T s = x[i];
-        for (int q2 = q + 1; q2 < cnt && G.out[e0 + q2] == dst; ++q2) s += x[q2 * size + e];
+        T *const dst = out_slot[q];
+        if (q > 0 && out_slot[q - 1] == dst) continue;
+        T s = x[px(i)];
+        for (int q2 = q + 1; q2 < cnt && out_slot[q2] == dst; ++q2) s += x[px(q2 * size + e)];
         atomicAdd((T *)(kronreg::as_global(dst) + e), s);
     }
     __syncthreads();  // LDS free for the next pass
@@ -115,6 +137,7 @@ template<typename T>
 __global__ void __launch_bounds__(kThreads) kron_group_kernel(GroupTable<T> const table)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ T *out_slot[kThreads];  // a pass holds at most kThreads entries (one fiber each at least)
     T *const lds = reinterpret_cast<T *>(smem_raw);
     for (int tile = blockIdx.x; tile < table.tiles; tile += gridDim.x)
     {
@@ -128,14 +151,14 @@ __global__ void __launch_bounds__(kThreads) kron_group_kernel(GroupTable<T> cons
             int const cnt = e_end - e0 < G.E ? e_end - e0 : G.E;
             switch (G.n)
             {
-            case 1: pass<T, 1>(G, e0, cnt, lds); break;
-            case 2: pass<T, 2>(G, e0, cnt, lds); break;
-            case 3: pass<T, 3>(G, e0, cnt, lds); break;
-            case 4: pass<T, 4>(G, e0, cnt, lds); break;
-            case 5: pass<T, 5>(G, e0, cnt, lds); break;
-            case 6: pass<T, 6>(G, e0, cnt, lds); break;
-            case 7: pass<T, 7>(G, e0, cnt, lds); break;
-            case 8: pass<T, 8>(G, e0, cnt, lds); break;
+            case 1: pass<T, 1>(G, e0, cnt, lds, out_slot); break;
+            case 2: pass<T, 2>(G, e0, cnt, lds, out_slot); break;
+            case 3: pass<T, 3>(G, e0, cnt, lds, out_slot); break;
+            case 4: pass<T, 4>(G, e0, cnt, lds, out_slot); break;
+            case 5: pass<T, 5>(G, e0, cnt, lds, out_slot); break;
+            case 6: pass<T, 6>(G, e0, cnt, lds, out_slot); break;
+            case 7: pass<T, 7>(G, e0, cnt, lds, out_slot); break;
+            case 8: pass<T, 8>(G, e0, cnt, lds, out_slot); break;
             default: break;  // rejected on the host
             }
         }

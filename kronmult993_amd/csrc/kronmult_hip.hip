@@ -787,11 +787,19 @@ int kronmult_adjoint_sync(int d, int n, T const *const *mats, int stride, T cons
 // ---- grouped launch -------------------------------------------------------------------------------------
 constexpr size_t kGroupLdsBudget = 64 * 1024;
 
+// LDS bytes one entry of a persistent pass takes (padded vector and factors, kron_group.hpp)
+template<typename T>
+long group_entry_bytes(Shape const &s)
+{
+    if (s.n > 8) return LONG_MAX;
+    return (static_cast<long>(krongroup::padded_size(s.n, static_cast<int>(s.size))) +
+            krongroup::coef_pitch(s.d, s.n)) * static_cast<long>(sizeof(T));
+}
+
 template<typename T, typename GroupT>
 bool group_persistent_ok(GroupT const &g, Shape const &s)
 {
-    long const per_entry = (s.size + static_cast<long>(s.d) * s.n * s.n) * static_cast<long>(sizeof(T));
-    return s.n <= 8 && per_entry <= static_cast<long>(kGroupLdsBudget);
+    return s.n <= 8 && group_entry_bytes<T>(s) <= static_cast<long>(kGroupLdsBudget);
 }
 
 template<typename T, typename GroupT>
@@ -863,7 +871,7 @@ hipError_t kronmult_grouped_launch(GroupT const *groups, int count, kronmult_gro
         D.nb = s.nb;
         D.size = static_cast<int>(s.size);
         D.nfib = static_cast<int>(s.size / s.n);
-        long const per_entry = (s.size + static_cast<long>(s.d) * s.n * s.n) * static_cast<long>(sizeof(T));
+        long const per_entry = group_entry_bytes<T>(s);
         int E = krongroup::kThreads / D.nfib;
         if (E < 1) E = 1;
         while (E > 1 && E * per_entry > static_cast<long>(kGroupLdsBudget)) --E;
