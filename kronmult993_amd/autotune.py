@@ -58,7 +58,7 @@ def entries_per_pass(d: int, n: int, itemsize: int) -> int:
     nfib = n ** (d - 1)
     size = n ** d
     padded = size + size // n if n in (4, 8) else size   # kron_group.hpp: padded rows, 16-byte factor pad
-    per_entry = (padded + d * n * n + 2) * itemsize
+    per_entry = (padded + d * n * n + 2) * itemsize + (d + 1) * 8   # + the pass's pointer slots
     e = max(1, GROUP_THREADS // nfib)
     while e > 1 and e * per_entry > LDS_BUDGET:
         e -= 1

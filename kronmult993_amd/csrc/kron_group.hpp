@@ -62,6 +62,12 @@ __host__ __device__ constexpr int coef_pitch(int d, int n)
 {
     return d * n * n + 2;
 }
+// LDS elements of type T holding one entry's d + 1 pointers
+template<typename T>
+__host__ __device__ constexpr int ptr_elems(int d)
+{
+    return static_cast<int>(((d + 1) * sizeof(void *) + sizeof(T) - 1) / sizeof(T));
+}
 
 // process entries [e0, e0 + cnt) of group G (cnt <= G.E) with N = G.n fixed at compile time
 template<typename T, int N>
@@ -73,14 +79,22 @@ __device__ __forceinline__ void pass(GroupDesc<T> const &G, int e0, int cnt, T *
     auto px = [](int i) { return pad_rows<N>() ? i + i / N : i; };  // element i of the entry-major vectors
     T *const x = lds;                                                           // cnt padded vectors
     T *const coef = lds + static_cast<size_t>(G.E) * padded_size(N, size);      // entry q at q * cp: a[m][r][k]
+    // the pass's input and factor pointers (E and E*d slots after the factors), read once, in parallel
+    constexpr int kPtrAlign = static_cast<int>(sizeof(void *) / sizeof(T));
+    size_t const ptr_off = (static_cast<size_t>(G.E) * (padded_size(N, size) + cp) + kPtrAlign - 1) / kPtrAlign * kPtrAlign;
+    T const **const ptr = reinterpret_cast<T const **>(lds + ptr_off);
     int const tid = threadIdx.x;
 
+    int const nptr = cnt * (d + 1);
+    for (int j = tid; j < nptr; j += blockDim.x)
+        ptr[j] = j < cnt ? G.in[e0 + j] : G.mats[static_cast<size_t>(e0) * d + (j - cnt)];
+    if (tid < cnt) out_slot[tid] = G.out[e0 + tid];  // the pass's output pointers
+    __syncthreads();
     for (int i = tid; i < cnt * size; i += blockDim.x)
     {
         int const q = i / size;
-        x[px(i)] = kronreg::as_global(G.in[e0 + q])[i - q * size];
+        x[px(i)] = kronreg::as_global(ptr[q])[i - q * size];
     }
-    if (tid < cnt) out_slot[tid] = G.out[e0 + tid];  // the pass's output pointers, read once
     int const ncoef = cnt * d * nn;
     for (int i = tid; i < ncoef; i += blockDim.x)
     {
@@ -88,8 +102,7 @@ __device__ __forceinline__ void pass(GroupDesc<T> const &G, int e0, int cnt, T *
         int const rk = i - qm * nn;
         int const r = rk / N, k = rk - (rk / N) * N;
         int const q = qm / d;
-        coef[q * cp + (qm - q * d) * nn + rk] =
-            kronreg::as_global(G.mats[static_cast<size_t>(e0) * d + qm])[r + k * G.stride];
+        coef[q * cp + (qm - q * d) * nn + rk] = kronreg::as_global(ptr[cnt + qm])[r + k * G.stride];
     }
     __syncthreads();
 

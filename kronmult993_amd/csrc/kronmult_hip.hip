@@ -793,7 +793,7 @@ long group_entry_bytes(Shape const &s)
 {
     if (s.n > 8) return LONG_MAX;
     return (static_cast<long>(krongroup::padded_size(s.n, static_cast<int>(s.size))) +
-            krongroup::coef_pitch(s.d, s.n)) * static_cast<long>(sizeof(T));
+            krongroup::coef_pitch(s.d, s.n) + krongroup::ptr_elems<T>(s.d)) * static_cast<long>(sizeof(T));
 }
 
 template<typename T, typename GroupT>
@@ -884,7 +884,7 @@ hipError_t kronmult_grouped_launch(GroupT const *groups, int count, kronmult_gro
         D.out = g.output_batched;
         table.tile_start[table.count] = tiles;
         tiles += (s.nb + D.epw - 1) / D.epw;
-        size_t const need = static_cast<size_t>(E) * per_entry;
+        size_t const need = static_cast<size_t>(E) * per_entry + sizeof(void *);  // + pointer alignment
         if (need > lds) lds = need;
         ++table.count;
     }
