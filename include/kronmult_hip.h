@@ -53,16 +53,20 @@ int kronmult_batched_f32_async(int matrix_count, int matrix_size, float const *c
  * Launch-shape control for the stream-ordered entry points (no counterpart in the reference, which has a
  * single launch shape, kronmult_gpu/kronmult.cu:184-193). Zero fields mean "library default".
  *   family:                 KRONMULT_FAMILY_AUTO, _REGISTER (register-resident sweep, only for the
- *                           (d, n) plans it is built for), _FIBER (fiber sweep, any n <= 64) or _MATRIX
- *                           (every contraction on the FP64 matrix cores; d = 6, n = 4, double only);
- *   workgroups_per_cu:      resident workgroups to launch per compute unit (register family);
- *   entries_per_workgroup:  consecutive entries one workgroup walks (register family).
+ *                           (d, n) plans it is built for), _FIBER (fiber sweep, one vector per
+ *                           workgroup, any n <= 64), _MATRIX (every contraction on the FP64 matrix cores;
+ *                           d = 6, n = 4, double only) or _PACKED (fiber sweep with several small vectors per workgroup: n <= 8,
+ *                           n^(d-1) <= 64; the default for such shapes from 20 entries per CU on);
+ *   workgroups_per_cu:      workgroups to launch per compute unit (register, matrix, packed families);
+ *   entries_per_workgroup:  consecutive entries one workgroup walks (register and matrix families), or
+ *                           entries per tile (packed family).
  * An impossible request (a family that does not cover (d, n)) returns hipErrorInvalidValue.
  */
 #define KRONMULT_FAMILY_AUTO 0
 #define KRONMULT_FAMILY_REGISTER 1
 #define KRONMULT_FAMILY_FIBER 2
 #define KRONMULT_FAMILY_MATRIX 3
+#define KRONMULT_FAMILY_PACKED 4
 
 typedef struct kronmult_tuning
 {
@@ -73,7 +77,7 @@ typedef struct kronmult_tuning
 
 typedef struct kronmult_launch_info
 {
-    int family;                /* KRONMULT_FAMILY_REGISTER, _FIBER or _MATRIX */
+    int family;                /* KRONMULT_FAMILY_REGISTER, _FIBER, _MATRIX or _PACKED */
     int grid;                  /* workgroups launched */
     int block;                 /* threads per workgroup */
     int entries_per_workgroup; /* entries walked by one workgroup */
@@ -112,7 +116,8 @@ int kronmult_hip_launch_info(int matrix_count, int matrix_size, int element_size
  * range, else S = P_0, S = S + P_1, ... in range order and out = out + S. Results are bitwise
  * reproducible for a given batch and launch shape. The synchronous entry points check the layout on the
  * host (copies of the output and workspace pointer arrays) and return hipErrorInvalidValue if it does not
- * hold; the _async ones trust the caller. Not available for KRONMULT_FAMILY_MATRIX.
+ * hold; the _async ones trust the caller. Not available for KRONMULT_FAMILY_MATRIX or _PACKED (the default
+ * for packed shapes falls back to _FIBER here).
  */
 int kronmult_batched_ordered_f64(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
                                  int matrix_stride, double **input_batched, double **output_batched,

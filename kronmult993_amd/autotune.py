@@ -8,7 +8,8 @@ variable KRONMULT_TUNE_CACHE).
 
 Choices:
   * one batch: the kernel family (register-resident sweep where a plan exists, matrix-core sweep for
-    d=6 n=4 FP64, fiber sweep) and, for the register and matrix-core families, workgroups per CU;
+    d=6 n=4 FP64, packed fiber sweep for small vectors, fiber sweep) and, for the register, matrix-core and
+    packed families, workgroups per CU;
   * several batches (a grouped call): the persistent launch -- workgroups per CU and per-group tile length
     (entries per tile, from a quarter of the entries the launch processes in one pass, which also shrinks
     the pass and its LDS, to four passes) -- against one launch per group with each group's own best
@@ -125,6 +126,11 @@ class Autotuner:
         try:
             api.launch_info(db.d, db.n, db.itemsize, db.nb, {"family": api.FAMILY_REGISTER})
             cands += [{"family": api.FAMILY_REGISTER, "workgroups_per_cu": w} for w in (1, 2, 4, 8, 16)]
+        except api.KronmultError:
+            pass
+        try:
+            api.launch_info(db.d, db.n, db.itemsize, db.nb, {"family": api.FAMILY_PACKED})
+            cands += [{"family": api.FAMILY_PACKED, "workgroups_per_cu": w} for w in (4, 8, 16)]
         except api.KronmultError:
             pass
         try:

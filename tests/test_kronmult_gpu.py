@@ -228,3 +228,27 @@ def test_matrix_core_family_rejects_other_shapes(gpu):
             api.launch_info(d, n, size, 100, tu)
         assert e.value.code == api.HIP_ERROR_INVALID_VALUE
     assert api.kernel_name(6, 4, 8, 100) == "reg_sweep_f64"
+
+
+@pytest.mark.parametrize("d,n", [(3, 4), (2, 8), (3, 6), (5, 2), (6, 2), (4, 3), (1, 8), (2, 2), (3, 7)])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_packed_family(gpu, d, n, dtype):
+    """Several small vectors per workgroup (the default for n^(d-1) <= 64 and large batches): padded strides,
+    shared factors, scattered aliasing; bit-identical per entry."""
+    b = W.random_case(120 + d * n, d, n, 6000, stride=n + 1, shared_factors=37).astype(dtype)
+    assert api.kernel_name(d, n, b.dtype.itemsize, b.nb).startswith("packed")
+    _check_aliased(b, _run(b, gpu))
+    u = _unaliased(b)
+    _check_bitexact(u, _run(u, gpu))
+
+
+@pytest.mark.parametrize("ept,wpc", [(1, 0), (3, 2), (0, 64)])
+def test_packed_family_launch_shapes(gpu, ept, wpc):
+    """Forced packed launches on a small batch with explicit tile lengths and grid sizes."""
+    tu = {"family": api.FAMILY_PACKED, "entries_per_workgroup": ept, "workgroups_per_cu": wpc}
+    b = W.random_case(131, 3, 4, 77, distinct=9)
+    _check_aliased(b, _run_tuned(b, gpu, tu))
+    u = _unaliased(b)
+    _check_bitexact(u, _run_tuned(u, gpu, tu))
+    with pytest.raises(api.KronmultError):
+        api.launch_info(4, 5, 8, 100, {"family": api.FAMILY_PACKED})   # 125 fibers: not packed
