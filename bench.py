@@ -31,6 +31,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# CPUs this process may run on, read before any OpenMP runtime starts: with threads bound to places, the
+# runtime pins the main thread to one core when it initialises, and the mask read afterwards holds one CPU
+try:
+    HOST_CPUS = len(os.sched_getaffinity(0))
+except AttributeError:
+    HOST_CPUS = os.cpu_count() or 1
+
 # OpenMP placement for the CPU baseline (read when the reference's OpenMP runtime starts): threads pinned to
 # cores, packed; the caller's settings win
 os.environ.setdefault("OMP_PROC_BIND", "close")
@@ -62,13 +69,9 @@ def log(msg: str) -> None:
 
 def cpu_threads() -> int:
     env = os.environ.get("OMP_NUM_THREADS")
-    try:
-        avail = len(os.sched_getaffinity(0))
-    except AttributeError:
-        avail = os.cpu_count() or 1
     if env and env.isdigit() and int(env) > 0:
-        return min(int(env), avail)
-    return avail
+        return min(int(env), HOST_CPUS)
+    return HOST_CPUS
 
 
 def cpu_baseline(hbs, runs: int):
@@ -100,13 +103,9 @@ def cpu_baseline(hbs, runs: int):
         log(f"[cpu] run {r} {'(warm-up) ' if r == 0 else ''}{(t1 - t0) * 1e3:.1f} ms on {threads} threads")
     med = statistics.median(times)
     flops = sum(hb.flops() for hb in hbs)
-    try:
-        avail = len(os.sched_getaffinity(0))
-    except AttributeError:
-        avail = os.cpu_count() or 1
     placement = (f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} OMP_PLACES={os.environ.get('OMP_PLACES')} "
-                 f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')}, {avail} CPUs in this process's affinity "
-                 f"mask, {os.cpu_count()} online")
+                 f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')}, {HOST_CPUS} CPUs in this process's "
+                 f"affinity mask at start, {os.cpu_count()} online")
     names = "+".join(hb.name for hb in hbs)
     nb = sum(hb.nb for hb in hbs)
     return {"value": round(flops / med / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": kind,
