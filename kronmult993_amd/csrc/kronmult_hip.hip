@@ -196,16 +196,17 @@ int device_cu_count()
     return v;
 }
 
-// Resident workgroups per CU of one kernel instantiation (cached; it depends only on the code object).
+// Resident workgroups per CU of one kernel instantiation (cached when the launch takes no dynamic LDS:
+// then it depends only on the code object; otherwise queried per call, the LDS varying with the shape).
 template<typename Kernel>
 int resident_per_cu(Kernel kernel, int block, size_t dyn_lds)
 {
     static std::atomic<int> cache{0};
-    int v = cache.load(std::memory_order_relaxed);
+    int v = dyn_lds == 0 ? cache.load(std::memory_order_relaxed) : 0;
     if (v == 0)
     {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, block, dyn_lds) != hipSuccess || v <= 0) v = 1;
-        cache.store(v, std::memory_order_relaxed);
+        if (dyn_lds == 0) cache.store(v, std::memory_order_relaxed);
     }
     return v;
 }
