@@ -96,8 +96,65 @@ __device__ __forceinline__ void mode_step(T *dst, T const *src, T const *A_gener
 // gA[r + c*stride] += sum over fibers f of H[base_f + r*st] * X[base_f + c*st]  (r, c < N).
 // red: LDS scratch of (blockDim.x / 64) * 256 elements (the 16 x 16 accumulator of every wavefront).
 template<typename T, int N>
-__device__ __forceinline__ void factor_gradient(T *gA, int stride, T const *H, T const *X, int st, int nfib, T *red)
+__device__ __forceinline__ void grad_vals(T const *H, T const *X, int st, int nfib, T *red, T (&out)[4])
 {
+    if constexpr (std::is_same<T, double>::value && (N <= 4 || N == 8))
+    {
+        // FP64, n <= 4 or n = 8: V_MFMA_F64_4X4X4_4B_F64, four 4 x 4 blocks per instruction; lane
+        // 16 kb + 4 jb + i supplies A_jb[i][kb] and B_jb[kb][i], D_jb[i][s] lands in lane 16 i + 4 jb + s.
+        // n <= 4: block jb sums its own fibers (16 fibers per instruction), the blocks are added by two lane
+        // shuffles. n = 8: block jb = (rb, cb) is quarter (rb, cb) of dA, 4 fibers per instruction.
+        int const lane = threadIdx.x & 63;
+        int const wave = threadIdx.x >> 6;
+        int const nwaves = blockDim.x >> 6;
+        int const i = lane & 3, jb = (lane >> 2) & 3, kb = lane >> 4;
+        double acc = 0.0;
+        if constexpr (N <= 4)
+        {
+            int const ngroups = (nfib + 15) >> 4;
+            for (int grp = wave; grp < ngroups; grp += nwaves)
+            {
+                int const f = grp * 16 + jb * 4 + kb;
+                double a = 0.0, b = 0.0;
+                if (i < N && f < nfib)
+                {
+                    int const base = fiber_base<N>(f, st);
+                    a = H[base + i * st];
+                    b = X[base + i * st];
+                }
+                acc = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc, 0, 0, 0);
+            }
+            acc += __shfl_xor(acc, 4);
+            acc += __shfl_xor(acc, 8);
+        }
+        else
+        {
+            int const rb = jb >> 1, cb = jb & 1;
+            int const ngroups = (nfib + 3) >> 2;
+            for (int grp = wave; grp < ngroups; grp += nwaves)
+            {
+                int const f = grp * 4 + kb;
+                double a = 0.0, b = 0.0;
+                if (f < nfib)
+                {
+                    int const base = fiber_base<N>(f, st);
+                    a = H[base + (4 * rb + i) * st];
+                    b = X[base + (4 * cb + i) * st];
+                }
+                acc = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc, 0, 0, 0);
+            }
+        }
+        if (nwaves > 1)
+        {
+            red[wave * 64 + lane] = acc;
+            __syncthreads();
+            if (wave == 0)
+                for (int w = 1; w < nwaves; ++w) acc += red[w * 64 + lane];
+        }
+        out[0] = acc;
+        out[1] = out[2] = out[3] = T(0);
+        return;
+    }
     using M = Mfma<T>;
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
@@ -131,15 +188,32 @@ __device__ __forceinline__ void factor_gradient(T *gA, int stride, T const *H, T
                 for (int v = 0; v < 4; ++v) acc[v] += red[w * 256 + v * 64 + lane];
         }
     }
-    if (wave == 0)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[v] = acc[v];
+}
+
+// Adds the factor gradient held by wavefront 0 (grad_vals layout) to gA.
+template<typename T, int N>
+__device__ __forceinline__ void grad_flush(T *gA, int stride, T const (&val)[4])
+{
+    int const lane = threadIdx.x & 63;
+    kronreg::global_ptr<T> g = kronreg::as_global(gA);
+    if constexpr (std::is_same<T, double>::value && (N <= 4 || N == 8))
     {
-        kronreg::global_ptr<T> g = kronreg::as_global(gA);
+        int const jb = (lane >> 2) & 3;
+        int const r = lane >> 4, c = lane & 3;  // lane 16 r + 4 jb + c holds block jb's [r][c]
+        int const row = N <= 4 ? r : 4 * (jb >> 1) + r;
+        int const col = N <= 4 ? c : 4 * (jb & 1) + c;
+        if (row < N && col < N && (N > 4 || jb == 0)) atomicAdd((T *)(g + row + col * stride), val[0]);
+    }
+    else
+    {
         int const c = lane & 15;
 #pragma unroll
         for (int v = 0; v < 4; ++v)
         {
-            int const r = M::row(lane, v);
-            if (r < N && c < N) atomicAdd((T *)(g + r + c * stride), acc[v]);
+            int const r = Mfma<T>::row(lane, v);
+            if (r < N && c < N) atomicAdd((T *)(g + r + c * stride), val[v]);
         }
     }
 }
@@ -147,14 +221,17 @@ __device__ __forceinline__ void factor_gradient(T *gA, int stride, T const *H, T
 // InLds: the working vectors live in LDS (dynamic shared memory), otherwise in ws[b].
 // Store: keep the chain X_0..X_{d-2} (d-1 vectors) instead of recomputing each X_t.
 // Scratch layout (elements): H [size], then Store ? X_0..X_{d-2} [(d-1) size] : X [size].
+// Factor gradients of up to kRunD factors are summed across the consecutive entries of a workgroup's run.
+constexpr int kRunD = 8;
+
 template<typename T, int N, bool InLds, bool Store>
-__global__ void __launch_bounds__(kThreads)
-    kron_adjoint_kernel(int d, int size, T const *const *__restrict__ mats, int stride, T const *const *__restrict__ in,
-                        T const *const *__restrict__ gout, T *const *__restrict__ gin, T *const *__restrict__ gmats,
-                        T *const *__restrict__ ws)
+__device__ __forceinline__ void kron_adjoint_entry(int b, int d, int size, T const *const *__restrict__ mats,
+                                                   int stride, T const *const *__restrict__ in,
+                                                   T const *const *__restrict__ gout, T *const *__restrict__ gin,
+                                                   T *const *__restrict__ gmats, T *const *__restrict__ ws,
+                                                   unsigned char *smem_raw, T (&dacc)[kRunD][4], T *(&dptr)[kRunD],
+                                                   bool wave0)
 {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    int const b = blockIdx.x;
     T const *const *A = mats + static_cast<size_t>(b) * d;
     T *const lds = reinterpret_cast<T *>(smem_raw);
     T *const H = InLds ? lds : ws[b];
@@ -206,7 +283,28 @@ __global__ void __launch_bounds__(kThreads)
                 }
                 Xt = X;
             }
-            factor_gradient<T, N>(gmats[static_cast<size_t>(b) * d + t], stride, H, Xt, st, nfib, red);
+            T vals[4];
+            grad_vals<T, N>(H, Xt, st, nfib, red, vals);
+            T *const gA = gmats[static_cast<size_t>(b) * d + t];
+            if (t < kRunD)
+            {
+#pragma unroll
+                for (int tt = 0; tt < kRunD; ++tt)
+                    if (tt == t)
+                    {
+                        if (dptr[tt] != gA)
+                        {
+                            if (wave0 && dptr[tt] != nullptr) grad_flush<T, N>(dptr[tt], stride, dacc[tt]);
+                            dptr[tt] = gA;
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) dacc[tt][v] = T(0);
+                        }
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) dacc[tt][v] += vals[v];
+                    }
+            }
+            else if (wave0)
+                grad_flush<T, N>(gA, stride, vals);
             __syncthreads();
         }
         if (t < d - 1 || want_input)
@@ -222,5 +320,44 @@ __global__ void __launch_bounds__(kThreads)
         for (int e = threadIdx.x; e < size; e += blockDim.x) atomicAdd((T *)(gi + e), H[e]);
     }
 }
+
+// A workgroup walks a run of entries [e0, e1) (logical run index as in the forward kernels). Factor
+// gradients of consecutive entries that write the same storage (shared factors) are summed in registers
+// of wavefront 0 -- for factors t < kRunD -- and added with one atomic per element when the storage
+// changes or the run ends.
+
+template<typename T, int N, bool InLds, bool Store>
+__global__ void __launch_bounds__(kThreads)
+    kron_adjoint_kernel(int nb, int epw, int d, int size, T const *const *__restrict__ mats, int stride,
+                        T const *const *__restrict__ in, T const *const *__restrict__ gout, T *const *__restrict__ gin,
+                        T *const *__restrict__ gmats, T *const *__restrict__ ws)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int const e0 = kronreg::xcd_run_index() * epw;
+    int const e1 = (e0 + epw < nb) ? e0 + epw : nb;
+    bool const wave0 = threadIdx.x < 64;
+    T dacc[kRunD][4];
+    T *dptr[kRunD];
+#pragma unroll
+    for (int tt = 0; tt < kRunD; ++tt)
+    {
+        dptr[tt] = nullptr;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dacc[tt][v] = T(0);
+    }
+    for (int b = e0; b < e1; ++b)
+    {
+        kron_adjoint_entry<T, N, InLds, Store>(b, d, size, mats, stride, in, gout, gin, gmats, ws, smem_raw, dacc,
+                                               dptr, wave0);
+        __syncthreads();  // the next entry overwrites the scratch
+    }
+    if (wave0)
+    {
+#pragma unroll
+        for (int tt = 0; tt < kRunD; ++tt)
+            if (dptr[tt] != nullptr) grad_flush<T, N>(dptr[tt], stride, dacc[tt]);
+    }
+}
+
 
 } // namespace kronadj

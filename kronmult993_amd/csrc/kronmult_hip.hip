@@ -830,8 +830,16 @@ template<typename T, int N, bool InLds, bool Store>
 hipError_t adjoint_launch_n(Shape const &s, AdjointPlan const &p, T const *const *mats, T const *const *in,
                             T const *const *gout, T *const *gin, T *const *gmats, T *const *ws, hipStream_t stream)
 {
-    hipLaunchKernelGGL((kronadj::kron_adjoint_kernel<T, N, InLds, Store>), dim3(s.nb), dim3(p.threads), p.lds,
-                       stream, s.d, static_cast<int>(s.size), mats, s.stride, in, gout, gin, gmats, ws);
+    // runs of entries, one per resident workgroup slot: consecutive entries that share factors sum their
+    // factor gradients in registers (the d=6 n=4 batch shares one factor block among all its entries)
+    auto kernel = kronadj::kron_adjoint_kernel<T, N, InLds, Store>;
+    long const slots = static_cast<long>(device_cu_count()) * resident_per_cu(kernel, p.threads, p.lds);
+    int epw = static_cast<int>((s.nb + slots - 1) / slots);
+    if (epw < 1) epw = 1;
+    long const runs = (s.nb + epw - 1) / epw;
+    long const grid = (runs + 7) / 8 * 8;  // surplus workgroups exit at once
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(grid)), dim3(p.threads), p.lds, stream, s.nb, epw, s.d,
+                       static_cast<int>(s.size), mats, s.stride, in, gout, gin, gmats, ws);
     return hipGetLastError();
 }
 
