@@ -54,7 +54,7 @@ def _check(b, gi, gm, want_input=True, want_mats=True):
         assert gm is None
 
 
-SHAPES = [(1, 1), (1, 5), (2, 3), (2, 8), (3, 5), (3, 8), (4, 4), (4, 8), (5, 3), (6, 2), (6, 4), (3, 7)]
+SHAPES = [(1, 1), (1, 5), (2, 3), (2, 8), (3, 5), (3, 8), (4, 4), (4, 8), (5, 3), (5, 4), (6, 2), (6, 4), (3, 7)]
 
 
 @pytest.mark.parametrize("intermediates", ["recompute", "store"])
@@ -66,7 +66,7 @@ def test_adjoint_random_f64(gpu, d, n, intermediates):
     _check(b, gi, gm)
 
 
-@pytest.mark.parametrize("d,n", [(2, 3), (3, 8), (4, 4), (6, 4)])
+@pytest.mark.parametrize("d,n", [(2, 3), (3, 8), (4, 4), (5, 4), (6, 4)])
 def test_adjoint_random_f32(gpu, d, n):
     b = W.random_case(seed=7 * d + n, d=d, n=n, nb=29, shared_factors=4, dtype=np.float32)
     gi, gm = _gpu_adjoint(b, gpu, "auto")
