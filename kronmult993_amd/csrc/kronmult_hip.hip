@@ -1395,7 +1395,7 @@ int kronmult_build_element_batch_f32(int matrix_count, int matrix_size, int base
 
 char const *kronmult_hip_version(void)
 {
-    return "kronmult_hip 0.3 (gfx950)";
+    return "kronmult_hip 0.4 (gfx950)";
 }
 
 char const *kronmult_hip_kernel_name(int matrix_count, int matrix_size, int element_size, int nb_batch)
