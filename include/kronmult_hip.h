@@ -174,7 +174,8 @@ int kronmult_hip_ordered_launch_info(int matrix_count, int matrix_size, int elem
  *
  * intermediates selects how the forward sweep's intermediate vectors are obtained:
  *   KRONMULT_ADJOINT_RECOMPUTE  rebuilt from the input for every factor (two n^d vectors of scratch);
- *   KRONMULT_ADJOINT_STORE      computed once and kept (d vectors of scratch);
+ *   KRONMULT_ADJOINT_STORE      computed once and kept (d vectors of scratch, or registers in the
+ *                               register-resident kernel where on-chip scratch would not fit);
  *   KRONMULT_ADJOINT_AUTO       the faster of the two where the library has measured it, else STORE when
  *                               its scratch fits in on-chip memory, else RECOMPUTE.
  * Scratch is on-chip when it fits; otherwise workspace_batched[k] must point to

@@ -265,10 +265,11 @@ struct Adj
         }
     }
 
-    // step t of the H chain (h in the layout of phase d-1-TT); X_TT from the kept x (RECOMPUTE) or the
-    // stored chain (STORE)
-    template<int TT, bool STORE>
-    static __device__ __forceinline__ void hstep(Ctx const &c, T (&h)[R], T const (&xr)[R], bool want_mats,
+    // step t of the H chain (h in the layout of phase d-1-TT); X_TT from the kept x (RECOMPUTE), the
+    // stored chain (STORE) or the chain kept in registers (KEEP: xk[m] = X_m in the layout of phase d-1-m;
+    // xk[d-1] = x in every mode)
+    template<int TT, bool STORE, bool KEEP>
+    static __device__ __forceinline__ void hstep(Ctx const &c, T (&h)[R], T const (&xk)[D][R], bool want_mats,
                                                  bool want_input)
     {
         constexpr int PH = D - 1 - TT;  // phase with mode TT in registers
@@ -284,9 +285,17 @@ struct Adj
             else
             {
                 T v[R];
+                if constexpr (KEEP)
+                {
 #pragma unroll
-                for (int j = 0; j < R; ++j) v[j] = xr[j];
-                xchain<0, PH>(c, v);
+                    for (int j = 0; j < R; ++j) v[j] = xk[TT][j];
+                }
+                else
+                {
+#pragma unroll
+                    for (int j = 0; j < R; ++j) v[j] = xk[D - 1][j];
+                    xchain<0, PH>(c, v);
+                }
                 lds_barrier();  // exchange reads (and previous readers of ht, ex) are done
                 store_fm(c.ex, v, c.t);
                 store_fm(c.ht, h, c.t);
@@ -302,7 +311,22 @@ struct Adj
         if constexpr (TT < D - 1)
         {
             exchange<PH, PH - 1>(c, h);
-            hstep<TT + 1, STORE>(c, h, xr, want_mats, want_input);
+            hstep<TT + 1, STORE, KEEP>(c, h, xk, want_mats, want_input);
+        }
+    }
+
+    // KEEP: X chain once, X_m for m = d-2 .. 0 kept in registers in the layout of phase d-1-m
+    template<int PH>
+    static __device__ __forceinline__ void xkeep(Ctx const &c, T (&v)[R], T (&xk)[D][R])
+    {
+        if constexpr (PH < D - 1)
+        {
+            int const m = D - 1 - PH;
+            contract<T, PL, false>(v, c.A[m], c.stride, coef_of(c, m));
+            exchange<PH, PH + 1>(c, v);
+#pragma unroll
+            for (int j = 0; j < R; ++j) xk[m - 1][j] = v[j];  // X_{m-1}, now with mode m-1 in registers
+            xkeep<PH + 1>(c, v, xk);
         }
     }
 
@@ -324,14 +348,15 @@ struct Adj
 // Waves per SIMD the register allocator must allow: the d=3 n=8 FP64 recompute kernel needs a few registers
 // more than three waves' share, and one more resident wave per SIMD is worth more than the spare registers
 // (the stored-chain kernel is held to two waves by its LDS anyway).
-template<typename T, typename PL, bool STORE>
+template<typename T, typename PL, bool STORE, bool KEEP>
 constexpr int adjoint_waves_per_simd()
 {
-    return (!STORE && PL::kD == 3 && PL::kN == 8 && sizeof(T) == 8) ? 3 : 1;
+    return (!STORE && !KEEP && PL::kD == 3 && PL::kN == 8 && sizeof(T) == 8) ? 3 : 1;
 }
 
-template<typename T, typename PL, bool STORE>
-__global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(adjoint_waves_per_simd<T, PL, STORE>())))
+template<typename T, typename PL, bool STORE, bool KEEP>
+__global__ void __launch_bounds__(PL::P)
+    __attribute__((amdgpu_waves_per_eu(adjoint_waves_per_simd<T, PL, STORE, KEEP>())))
     kron_reg_adjoint_kernel(int nb, int epw, T const *const *__restrict__ mats, int stride,
                             T const *const *__restrict__ in, T const *const *__restrict__ gout,
                             T *const *__restrict__ gin, T *const *__restrict__ gmats)
@@ -377,11 +402,11 @@ __global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(adjo
 
     for (int e = e0; e < e1; ++e)
     {
-        T xr[R], h[R];
+        T xk[D][R], h[R];  // xk[d-1]: x in the phase-0 layout
 #pragma unroll
         for (int j = 0; j < R; ++j)
         {
-            xr[j] = xpre[j];
+            xk[D - 1][j] = xpre[j];
             h[j] = gpre[j];
         }
         lds_barrier();  // the previous entry is done with every LDS region
@@ -389,19 +414,22 @@ __global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(adjo
         if (e + 1 < e1) prefetch(e + 1);
         c.A = mats + static_cast<size_t>(e) * D;
         c.gA = want_mats ? gmats + static_cast<size_t>(e) * D : nullptr;
-        if constexpr (STORE)
+        if constexpr (STORE || KEEP)
         {
             if (want_mats)
             {
                 T v[R];
 #pragma unroll
-                for (int j = 0; j < R; ++j) v[j] = xr[j];
+                for (int j = 0; j < R; ++j) v[j] = xk[D - 1][j];
                 lds_barrier();  // staged factors visible
-                A::template xstore<0>(c, v);
+                if constexpr (STORE)
+                    A::template xstore<0>(c, v);
+                else
+                    A::template xkeep<0>(c, v, xk);
             }
         }
         lds_barrier();  // staged factors (and stored chain) visible
-        A::template hstep<0, STORE>(c, h, xr, want_mats, want_input);
+        A::template hstep<0, STORE, KEEP>(c, h, xk, want_mats, want_input);
         if (want_input)
         {
             // h = d x in the phase-0 layout (elements t*R + j): through LDS into the phase-(d-1) layout

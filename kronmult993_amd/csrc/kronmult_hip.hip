@@ -785,16 +785,23 @@ bool reg_adjoint_store_ok(Shape const &s)
 }
 
 template<typename T>
-bool reg_adjoint_ok(Shape const &s, int intermediates)
+bool reg_adjoint_ok(Shape const &s, int /*intermediates*/)
 {
-    bool const plan = (s.d == 3 && s.n == 8) || (s.d == 4 && s.n == 4) || (s.d == 4 && s.n == 8) ||
-                      (s.d == 5 && s.n == 4) || (s.d == 6 && s.n == 4);
-    if (!plan) return false;
-    return intermediates != KRONMULT_ADJOINT_STORE || reg_adjoint_store_ok<T>(s);
+    return (s.d == 3 && s.n == 8) || (s.d == 4 && s.n == 4) || (s.d == 4 && s.n == 8) || (s.d == 5 && s.n == 4) ||
+           (s.d == 6 && s.n == 4);
 }
 
+// How the register kernel obtains X_t: rebuilt from x for every t, the chain kept fiber-major in LDS, or the
+// chain kept in registers (each X_t in the layout of the phase that holds mode t: (d-1) R more values).
+enum class RegAdjMode
+{
+    Recompute,
+    StoreLds,
+    KeepRegisters
+};
+
 template<typename T, int D, int N>
-hipError_t reg_adjoint_launch(Shape const &s, bool store, T const *const *mats, T const *const *in,
+hipError_t reg_adjoint_launch(Shape const &s, RegAdjMode mode, T const *const *mats, T const *const *in,
                               T const *const *gout, T *const *gin, T *const *gmats, hipStream_t stream)
 {
     using PL = kronplan::Plan<D, N, 1>;
@@ -806,34 +813,53 @@ hipError_t reg_adjoint_launch(Shape const &s, bool store, T const *const *mats, 
     if (epw < 1) epw = 1;
     long const runs = (s.nb + epw - 1) / epw;
     long const grid = (runs + 7) / 8 * 8;
-    if (store)
+    dim3 const g(static_cast<unsigned>(grid)), blk(PL::P);
+    switch (mode)
     {
+    case RegAdjMode::StoreLds:
         if constexpr ((D == 3 && N == 8) || (D == 4 && N == 4))
         {
-            hipLaunchKernelGGL((kronregadj::kron_reg_adjoint_kernel<T, PL, true>), dim3(static_cast<unsigned>(grid)),
-                               dim3(PL::P), 0, stream, s.nb, epw, mats, s.stride, in, gout, gin, gmats);
+            hipLaunchKernelGGL((kronregadj::kron_reg_adjoint_kernel<T, PL, true, false>), g, blk, 0, stream, s.nb,
+                               epw, mats, s.stride, in, gout, gin, gmats);
             return hipGetLastError();
         }
         return hipErrorInvalidValue;
+    case RegAdjMode::KeepRegisters:
+        hipLaunchKernelGGL((kronregadj::kron_reg_adjoint_kernel<T, PL, false, true>), g, blk, 0, stream, s.nb, epw,
+                           mats, s.stride, in, gout, gin, gmats);
+        return hipGetLastError();
+    default:
+        hipLaunchKernelGGL((kronregadj::kron_reg_adjoint_kernel<T, PL, false, false>), g, blk, 0, stream, s.nb,
+                           epw, mats, s.stride, in, gout, gin, gmats);
+        return hipGetLastError();
     }
-    hipLaunchKernelGGL((kronregadj::kron_reg_adjoint_kernel<T, PL, false>), dim3(static_cast<unsigned>(grid)),
-                       dim3(PL::P), 0, stream, s.nb, epw, mats, s.stride, in, gout, gin, gmats);
-    return hipGetLastError();
 }
 
 template<typename T>
 hipError_t reg_adjoint_dispatch(Shape const &s, int intermediates, T const *const *mats, T const *const *in,
                                 T const *const *gout, T *const *gin, T *const *gmats, hipStream_t stream)
 {
-    // AUTO: FP32 keeps the chain (its kernel has registers and LDS to spare); FP64 recomputes, which keeps
-    // three waves per SIMD resident (measured on the d=3 n=8 batch: 41.7 us against 46.1 us)
-    bool const store = intermediates == KRONMULT_ADJOINT_STORE ||
-                       (intermediates == KRONMULT_ADJOINT_AUTO && sizeof(T) == 4 && reg_adjoint_store_ok<T>(s));
-    if (s.d == 3 && s.n == 8) return reg_adjoint_launch<T, 3, 8>(s, store, mats, in, gout, gin, gmats, stream);
-    if (s.d == 5 && s.n == 4) return reg_adjoint_launch<T, 5, 4>(s, store, mats, in, gout, gin, gmats, stream);
-    if (s.d == 6 && s.n == 4) return reg_adjoint_launch<T, 6, 4>(s, store, mats, in, gout, gin, gmats, stream);
-    if (s.d == 4 && s.n == 4) return reg_adjoint_launch<T, 4, 4>(s, store, mats, in, gout, gin, gmats, stream);
-    if (s.d == 4 && s.n == 8) return reg_adjoint_launch<T, 4, 8>(s, store, mats, in, gout, gin, gmats, stream);
+    // STORE: the chain in LDS where it fits comfortably (d=3 n=8, d=4 n=4), else in registers.
+    // AUTO, measured per shape: d=4 n=8 and d=5/6 n=4 keep the chain in registers (d=6 n=4 FP64 0.58 ms
+    // against 0.78 ms recomputing, FP32 0.38 against 0.49; d=4 n=8 FP64 1.01 against 1.15 ms); d=3 n=8 and
+    // d=4 n=4 recompute in FP64 (three waves per SIMD: 41.7 us against 46.4 us keeping the chain at the d=3
+    // n=8 batch) and keep the chain in LDS in FP32 (25.9 us)
+    bool const lds_ok = reg_adjoint_store_ok<T>(s);
+    bool const keep_preferred = (s.d == 4 && s.n == 8) || (s.d == 5 && s.n == 4) || (s.d == 6 && s.n == 4);
+    RegAdjMode mode;
+    if (intermediates == KRONMULT_ADJOINT_RECOMPUTE)
+        mode = RegAdjMode::Recompute;
+    else if (intermediates == KRONMULT_ADJOINT_STORE)
+        mode = lds_ok ? RegAdjMode::StoreLds : RegAdjMode::KeepRegisters;
+    else if (keep_preferred)
+        mode = RegAdjMode::KeepRegisters;
+    else
+        mode = (sizeof(T) == 4 && lds_ok) ? RegAdjMode::StoreLds : RegAdjMode::Recompute;
+    if (s.d == 3 && s.n == 8) return reg_adjoint_launch<T, 3, 8>(s, mode, mats, in, gout, gin, gmats, stream);
+    if (s.d == 5 && s.n == 4) return reg_adjoint_launch<T, 5, 4>(s, mode, mats, in, gout, gin, gmats, stream);
+    if (s.d == 6 && s.n == 4) return reg_adjoint_launch<T, 6, 4>(s, mode, mats, in, gout, gin, gmats, stream);
+    if (s.d == 4 && s.n == 4) return reg_adjoint_launch<T, 4, 4>(s, mode, mats, in, gout, gin, gmats, stream);
+    if (s.d == 4 && s.n == 8) return reg_adjoint_launch<T, 4, 8>(s, mode, mats, in, gout, gin, gmats, stream);
     return hipErrorInvalidValue;
 }
 
@@ -1343,7 +1369,8 @@ long long kronmult_batched_adjoint_workspace(int matrix_count, int matrix_size, 
     if (validate(matrix_count, matrix_size, matrix_size, 0, s) != hipSuccess) return -1;
     AdjointPlan p;
     if (adjoint_plan(s, element_size, intermediates, device_max_lds(), p) != hipSuccess) return -1;
-    return p.ws_elems;
+    bool const reg = element_size == 8 ? reg_adjoint_ok<double>(s, intermediates) : reg_adjoint_ok<float>(s, intermediates);
+    return reg ? 0 : p.ws_elems;  // the register-resident kernel keeps its scratch on chip
 }
 
 int kronmult_batched_grouped_f64(kronmult_group_f64_t const *groups, int group_count,
