@@ -131,6 +131,15 @@ constexpr bool coef_in_lds()
     return PL::kN >= 8;
 }
 
+// The forward sweep's choice: FP32 factors of n = 8 (64 values) still fit in scalar registers, and scalar
+// loads measured faster there (d=3 n=8: 9.8 against 10.7 us; d=4 n=8: 190 against 196 us); FP64 ones
+// (128 registers) are staged in LDS (scalar: 22.2 against 17.9 us, 320 against 299 us).
+template<typename T, typename PL>
+constexpr bool fwd_coef_in_lds()
+{
+    return coef_in_lds<PL>() && sizeof(T) == 8;
+}
+
 template<typename T, typename PL>
 struct Coef
 {
@@ -170,7 +179,7 @@ __device__ __forceinline__ void contract_phase(T (&v)[PL::R], T const *const *__
 {
     if constexpr (M >= PL::lo(PH))
     {
-        if constexpr (coef_in_lds<PL>())
+        if constexpr (fwd_coef_in_lds<T, PL>())
             contract_mode_lds<T, PL::kN, PL::R, PL::reg_stride(M - PL::lo(PH))>(v, coef + M * PL::kN * PL::kN);
         else
             contract_mode<T, PL::kN, PL::R, PL::reg_stride(M - PL::lo(PH))>(v, A_list[M], stride);
@@ -367,7 +376,7 @@ __global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(min_
                     T *const *__restrict__ out, T *const *__restrict__ ws)
 {
     static_assert(PL::NPHASE >= 1, "");
-    constexpr bool kCoefLds = coef_in_lds<PL>();
+    constexpr bool kCoefLds = fwd_coef_in_lds<T, PL>();
     constexpr int kCoefSlots = kCoefLds ? Coef<T, PL>::kCount : 0;
     __shared__ __attribute__((aligned(16))) T lds[PL::SIZE + kCoefSlots];
     T *coef = lds + PL::SIZE;
@@ -432,7 +441,7 @@ __global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(2)))
                           T const *const *__restrict__ in, T *const *__restrict__ out)
 {
     static_assert(PL::NPHASE >= 1, "");
-    constexpr bool kCoefLds = coef_in_lds<PL>();
+    constexpr bool kCoefLds = fwd_coef_in_lds<T, PL>();
     constexpr int kCoefSlots = kCoefLds ? Coef<T, PL>::kCount : 0;
     __shared__ __attribute__((aligned(16))) T lds[PL::SIZE + kCoefSlots];
     T *coef = lds + PL::SIZE;

@@ -240,7 +240,7 @@ hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T 
     auto kernel = s.ordered ? kronreg::kron_reg_kernel<T, PL, true> : kronreg::kron_reg_kernel<T, PL, false>;
     L.family = KRONMULT_FAMILY_REGISTER;
     L.block = PL::P;
-    L.lds = sizeof(T) * (PL::SIZE + (kronreg::coef_in_lds<PL>() ? kronreg::Coef<T, PL>::kCount : 0));
+    L.lds = sizeof(T) * (PL::SIZE + (kronreg::fwd_coef_in_lds<T, PL>() ? kronreg::Coef<T, PL>::kCount : 0));
     L.resident = resident_per_cu(kernel, PL::P, 0);
     int const per_cu = tu.workgroups_per_cu > 0 ? tu.workgroups_per_cu : default_workgroups_per_cu(PL::P);
     long const slots = static_cast<long>(device_cu_count()) * per_cu;
