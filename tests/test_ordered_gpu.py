@@ -99,3 +99,28 @@ def test_ordered_rejects_bad_layouts(gpu):
     torch.cuda.synchronize()
     assert np.array_equal(db.output(), before)
     api.kronmult_batched_ordered(*args, dg.ws_ptrs, 0, dtype="f64")   # nb = 0: nothing to do
+
+
+def test_ordered_graph_replays_are_bitwise_identical(gpu):
+    """The stream-ordered ordered call captured in a graph (sweep + reduction kernels, nothing allocated):
+    every replay, with eager output resets between replays, reproduces the eager result bit for bit."""
+    import torch
+    b = W.c2(rows=6, cols=64)
+    db = DeviceBatch(b, device=str(gpu), workspace="per_entry")
+    db.run_ordered(sync=False)
+    torch.cuda.synchronize()
+    eager = db.output()
+    assert np.array_equal(eager, ordered_reference(b, api.ordered_launch_info(6, 4, 8, b.nb)["entries_per_workgroup"]))
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        db.reset_output()
+        s.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            db.run_ordered(sync=False, stream=s)
+    for replay in range(3):
+        db.reset_output()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(db.output(), eager), f"replay {replay} differs"
