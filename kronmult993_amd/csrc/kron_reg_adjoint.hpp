@@ -33,6 +33,15 @@ using kronreg::constant_ptr;
 using kronreg::global_ptr;
 using kronreg::lds_barrier;
 
+// Factors staged in LDS (n = 8) or read with scalar loads: the adjoint follows the forward's rule except
+// for FP32 d=4 n=8, whose register-kept chain leaves too few scalar registers (measured 0.74 against 0.68 ms
+// with the factors in LDS); FP32 d=3 n=8 gains from scalar loads (24.6 against 26.1 us).
+template<typename T, typename PL>
+constexpr bool adj_coef_in_lds()
+{
+    return kronreg::coef_in_lds<PL>() && (sizeof(T) == 8 || PL::kD != 3);
+}
+
 template<typename T, typename PL>
 struct AdjShape
 {
@@ -46,7 +55,7 @@ struct AdjShape
     static constexpr int NSUB = P / NN;  // fiber subsets of the factor-gradient sums
     static constexpr int FS = NFIB / NSUB;
     static constexpr int EX = SIZE > FM ? SIZE : FM;
-    static constexpr bool kCoefLds = kronreg::coef_in_lds<PL>();
+    static constexpr bool kCoefLds = adj_coef_in_lds<T, PL>();
     static constexpr int COEF = kCoefLds ? kronreg::Coef<T, PL>::kCount : 0;
     static constexpr int RED = NSUB > 1 ? NSUB * NN : 0;
     static_assert(P % NN == 0 && FS % G == 0, "factor-gradient subsets must tile the fibers");
@@ -62,7 +71,7 @@ __device__ __forceinline__ void contract(T (&v)[PL::R], T const *A_generic, int 
     T x[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) x[k] = v[k];
-    if constexpr (kronreg::coef_in_lds<PL>())
+    if constexpr (adj_coef_in_lds<T, PL>())
     {
 #pragma unroll
         for (int r = 0; r < N; ++r)

@@ -841,9 +841,9 @@ hipError_t reg_adjoint_dispatch(Shape const &s, int intermediates, T const *cons
 {
     // STORE: the chain in LDS where it fits comfortably (d=3 n=8, d=4 n=4), else in registers.
     // AUTO, measured per shape: d=4 n=8 and d=5/6 n=4 keep the chain in registers (d=6 n=4 FP64 0.58 ms
-    // against 0.78 ms recomputing, FP32 0.38 against 0.49; d=4 n=8 FP64 1.01 against 1.15 ms); d=3 n=8 and
-    // d=4 n=4 recompute in FP64 (three waves per SIMD: 41.7 us against 46.4 us keeping the chain at the d=3
-    // n=8 batch) and keep the chain in LDS in FP32 (25.9 us)
+    // against 0.78 ms recomputing, FP32 0.38 against 0.49; d=4 n=8 FP64 1.01 against 1.15 ms); d=3 n=8
+    // recomputes (FP64 three waves per SIMD: 41.7 us against 46.4 us keeping the chain; FP32 24.6 us
+    // against 24.9 us with the chain in LDS); d=4 n=4 recomputes in FP64, keeps the chain in LDS in FP32
     bool const lds_ok = reg_adjoint_store_ok<T>(s);
     bool const keep_preferred = (s.d == 4 && s.n == 8) || (s.d == 5 && s.n == 4) || (s.d == 6 && s.n == 4);
     RegAdjMode mode;
@@ -854,7 +854,7 @@ hipError_t reg_adjoint_dispatch(Shape const &s, int intermediates, T const *cons
     else if (keep_preferred)
         mode = RegAdjMode::KeepRegisters;
     else
-        mode = (sizeof(T) == 4 && lds_ok) ? RegAdjMode::StoreLds : RegAdjMode::Recompute;
+        mode = (sizeof(T) == 4 && lds_ok && !(s.d == 3 && s.n == 8)) ? RegAdjMode::StoreLds : RegAdjMode::Recompute;
     if (s.d == 3 && s.n == 8) return reg_adjoint_launch<T, 3, 8>(s, mode, mats, in, gout, gin, gmats, stream);
     if (s.d == 5 && s.n == 4) return reg_adjoint_launch<T, 5, 4>(s, mode, mats, in, gout, gin, gmats, stream);
     if (s.d == 6 && s.n == 4) return reg_adjoint_launch<T, 6, 4>(s, mode, mats, in, gout, gin, gmats, stream);
