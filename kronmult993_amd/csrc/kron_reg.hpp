@@ -3,7 +3,8 @@
 // One workgroup of P = n^(d-RHO) threads walks a contiguous run of batch entries. Per entry:
 //   1. the input vector, prefetched into registers one entry ahead with 16-byte coalesced loads, is stored
 //      to LDS (swizzled canonical order) and read back in the phase-0 layout (each thread: one row of
-//      R = n^RHO consecutive elements, i.e. the RHO fastest modes);
+//      R = n^RHO consecutive elements, i.e. the RHO fastest modes); the FP64 d = 5, 6 n = 4 plans load it
+//      straight into LDS instead (LDS-DMA, issued once the previous entry's last phase has been read);
 //   2. each phase contracts the modes of its register block that are due, fastest first, entirely in
 //      registers: an n-term fused multiply-add chain per output, k ascending from zero, coefficients
 //      uniform across the workgroup (scalar loads) -- bit-identical to the reference's step
@@ -217,6 +218,51 @@ struct Lds
         }
     }
 
+    // Input staged by the LDS-DMA load (GLOBAL_LOAD_LDS_DWORDX4): lane l of wavefront w writes LDS granule
+    // base + l, where base = c*P + 64*w for instruction c, so the lane fetches the input granule whose
+    // swizzled place that is. The input map only XORs higher granule bits into bits 0..3, so it is its own
+    // inverse and each 16-lane group reads a permutation of 16 consecutive granules (two whole lines).
+    // The data bypasses the registers; the loads count against the vector memory counter.
+    static constexpr bool dma_ok()
+    {
+        constexpr unsigned long long m = kronplan::swizzle_masks(PL::kD, PL::kN, PL::kRho, static_cast<int>(sizeof(T)), 0);
+        return kVec && PL::P % 64 == 0 && (m & 0x000f000f000f000full) == 0;
+    }
+
+    static __device__ __forceinline__ void dma_input(T *lds, T const *src, int t)
+    {
+        static_assert(dma_ok(), "the input map must be an involution on whole wavefront blocks");
+        int const wbase = __builtin_amdgcn_readfirstlane(t & ~63);
+        int const lane = t & 63;
+#pragma unroll
+        for (int c = 0; c < PL::R / G; ++c)
+        {
+            int const p = c * PL::P + wbase + lane;  // physical granule written by this lane
+            int const q = SW_IN::phys(p * G) / G;    // logical granule that belongs there
+            auto g = (__attribute__((address_space(1))) void *)(src + q * G);
+            auto l = (__attribute__((address_space(3))) void *)(lds + (c * PL::P + wbase) * G);
+            __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
+        }
+    }
+
+    // Phase-0 read of a DMA-staged input. Thread t's row is granules R/G*t + q; the input map leaves
+    // granule bits 0..3 out of its parities, so granule q sits at phys(row) ^ q: one base register and an
+    // XOR per load. The empty asm makes the base opaque inside the entry loop, so the compiler computes
+    // the R/G addresses there instead of keeping (or spilling) them across the whole loop.
+    static __device__ __forceinline__ void read0_dma(T const *lds, T (&v)[PL::R], int t)
+    {
+        static_assert(dma_ok() && PL::R / G <= 16, "granule index must stay below the mapped bits");
+        int b = SW_IN::phys(PL::elem(0, t, 0));
+        asm volatile("" : "+v"(b));
+#pragma unroll
+        for (int q = 0; q < PL::R / G; ++q)
+        {
+            V g = *reinterpret_cast<V const *>(lds + (b ^ (q * G)));
+#pragma unroll
+            for (int i = 0; i < G; ++i) v[q * G + i] = g[i];
+        }
+    }
+
     template<int PH>
     static __device__ __forceinline__ void read(T const *lds, T (&v)[PL::R], int t)
     {
@@ -295,10 +341,18 @@ __device__ __forceinline__ void load_input(T const *src_generic, T (&pre)[PL::R]
     }
 }
 
-template<typename T, typename PL, int PH>
-__device__ __forceinline__ void run_phases(T *lds, T (&v)[PL::R], T const *const *__restrict__ A_list, int stride,
-                                           T const *coef, int t)
+struct NoHook
 {
+    __device__ void operator()() const {}
+};
+
+// the phases from PH on; after_last_read runs once the last phase's values are in registers (before its
+// contraction), when the LDS tensor is no longer needed by this thread
+template<typename T, typename PL, int PH, typename F = NoHook>
+__device__ __forceinline__ void run_phases(T *lds, T (&v)[PL::R], T const *const *__restrict__ A_list, int stride,
+                                           T const *coef, int t, F const &after_last_read = F())
+{
+    if constexpr (PH + 1 == PL::NPHASE) after_last_read();
     contract_phase<T, PL, PH, PL::c_hi(PH) - 1>(v, A_list, stride, coef);
     if constexpr (PH + 1 < PL::NPHASE)
     {
@@ -306,7 +360,7 @@ __device__ __forceinline__ void run_phases(T *lds, T (&v)[PL::R], T const *const
         Lds<T, PL>::template write<PH>(lds, v, t);
         lds_barrier();
         Lds<T, PL>::template read<PH + 1>(lds, v, t);
-        run_phases<T, PL, PH + 1>(lds, v, A_list, stride, coef, t);
+        run_phases<T, PL, PH + 1>(lds, v, A_list, stride, coef, t, after_last_read);
     }
 }
 
@@ -361,12 +415,28 @@ __device__ __forceinline__ int xcd_run_index()
     return (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
 }
 
+
+// S_WAITCNT immediate for vmcnt(0) with the export and LDS counters left alone (gfx9 encoding: vmcnt in
+// bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+constexpr int kWaitVmcnt0 = 0x0F70;
+
+// Input staging by LDS-DMA (Lds::dma_input): the FP64 n = 4 two-modes-per-phase plans (d = 5, 6), whose
+// factors come through scalar registers; frees the 16 prefetch values per thread.
+template<typename T, typename PL>
+constexpr bool dma_staging()
+{
+    return sizeof(T) == 8 && PL::kN == 4 && PL::kRho == 2 && PL::NPHASE >= 2 && !fwd_coef_in_lds<T, PL>() &&
+           Lds<T, PL>::dma_ok();
+}
+
 // Waves per SIMD the register allocator must allow (the kernel's VGPR budget is 512 / waves): chosen so
-// that an extra workgroup fits per CU where the plan is a few registers over a threshold.
+// that an extra workgroup fits per CU where the plan is a few registers over a threshold. The DMA-staged
+// d=6 n=4 FP64 sweep fits 128 registers once the prefetch registers are gone (4 workgroups per CU: 49.5
+// against 53.3 us at 3 and 55.6 us with register staging); three was the best without DMA (168 registers).
 template<typename T, typename PL>
 constexpr int min_waves_per_simd()
 {
-    return (PL::kD == 6 && PL::kN == 4 && sizeof(T) == 8) ? 3 : (PL::kD == 5 && PL::kN == 4 && sizeof(T) == 8) ? 3 : 1;
+    return dma_staging<T, PL>() ? 4 : 1;
 }
 
 // ORDERED: the accumulator is flushed per piece (flush_piece) instead of with atomics; ws is read only then.
@@ -388,13 +458,61 @@ __global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(min_
     T acc[PL::R];
 #pragma unroll
     for (int j = 0; j < PL::R; ++j) acc[j] = T(0);
-    T pre[PL::R];
-    T pre_coef[kCoefLds ? Coef<T, PL>::kPerThread : 1];
     T v[PL::R];
-    load_input<T, PL>(in[e0], pre, t);
-    if constexpr (kCoefLds) Coef<T, PL>::load(mats + static_cast<size_t>(e0) * PL::kD, stride, pre_coef, t);
     T *cur = out[e0];
     int ps = e0;  // first entry of the current piece (entries with output cur)
+
+    if constexpr (dma_staging<T, PL>())
+    {
+        // The next entry's input is loaded straight into LDS once every thread has read the last phase
+        // of the current one; a misaligned input pointer is staged through registers instead.
+        auto issue = [&](int e) -> bool {
+            T const *src = in[e];
+            if ((reinterpret_cast<uintptr_t>(src) & 15) != 0) return false;
+            Lds<T, PL>::dma_input(lds, src, t);
+            return true;
+        };
+        bool staged = issue(e0);
+        for (int e = e0; e < e1; ++e)
+        {
+            if (!staged)
+            {
+                T tmp[PL::R];
+                load_input<T, PL>(in[e], tmp, t);
+                Lds<T, PL>::store_input(lds, tmp, t);
+            }
+            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);  // this wavefront's LDS-DMA loads have landed
+            lds_barrier();
+            Lds<T, PL>::read0_dma(lds, v, t);
+            run_phases<T, PL, 0>(lds, v, mats + static_cast<size_t>(e) * PL::kD, stride, coef, t, [&]() {
+                lds_barrier();  // every thread has read the last phase: LDS free
+                staged = (e + 1 < e1) ? issue(e + 1) : true;
+            });
+
+            T *dst = out[e];
+            if (dst != cur)
+            {
+                if constexpr (ORDERED)
+                    flush_piece<T, PL>(cur, acc, t, out, ws, nb, ps, e);
+                else
+                    flush<T, PL>(cur, acc, t);
+                cur = dst;
+                ps = e;
+            }
+#pragma unroll
+            for (int j = 0; j < PL::R; ++j) acc[j] += v[j];
+        }
+        if constexpr (ORDERED)
+            flush_piece<T, PL>(cur, acc, t, out, ws, nb, ps, e1);
+        else
+            flush<T, PL>(cur, acc, t);
+        return;
+    }
+
+    T pre[PL::R];
+    T pre_coef[kCoefLds ? Coef<T, PL>::kPerThread : 1];
+    load_input<T, PL>(in[e0], pre, t);
+    if constexpr (kCoefLds) Coef<T, PL>::load(mats + static_cast<size_t>(e0) * PL::kD, stride, pre_coef, t);
 
     for (int e = e0; e < e1; ++e)
     {

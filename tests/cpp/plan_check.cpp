@@ -107,6 +107,49 @@ void check_plan(char const *name, bool conflict_free)
                 CHECK(phys(e + k) == phys(e) + k, "%s swizzle splits a granule at %d", name, e);
     }
 
+    // 2b. LDS-DMA input staging (kron_reg.hpp Lds::dma_input / read0_dma), for the plans whose input map
+    // leaves granule bits 0..3 out of its parities: lane l of wavefront w writes LDS granule c*P + 64w + l
+    // and fetches the input granule phys^-1 of it (= phys of it, an involution); every input granule must
+    // be fetched exactly once, each 16-lane group must fetch 16 consecutive granules (whole lines), and
+    // the phase-0 row of thread t must sit at phys(row) ^ q.
+    {
+        unsigned long long const m = swizzle_masks(PL::kD, PL::kN, PL::kRho, static_cast<int>(sizeof(T)), 0);
+        if (R >= G && P % 64 == 0 && (m & 0x000f000f000f000full) == 0)
+        {
+            int const NG = S / G;
+            std::vector<int> fetched(NG, 0);
+            bool coalesced = true, involution = true;
+            for (int c = 0; c < R / G; ++c)
+                for (int w = 0; w < P / 64; ++w)
+                    for (int l0 = 0; l0 < 64; l0 += 16)
+                    {
+                        std::set<int> grp;
+                        for (int l = l0; l < l0 + 16; ++l)
+                        {
+                            int const p = c * P + 64 * w + l;
+                            int const q = SW_IN::phys(p * G) / G;
+                            involution = involution && SW_IN::phys(q * G) / G == p;
+                            if (q >= 0 && q < NG) fetched[q]++;
+                            grp.insert(q);
+                        }
+                        coalesced = coalesced && grp.size() == 16 && *grp.begin() % 16 == 0 &&
+                                    *grp.rbegin() == *grp.begin() + 15;
+                    }
+            bool once = true;
+            for (int g = 0; g < NG; ++g) once = once && fetched[g] == 1;
+            CHECK(involution, "%s DMA: input map is not an involution", name);
+            CHECK(once, "%s DMA: input granules not fetched exactly once", name);
+            CHECK(coalesced, "%s DMA: a 16-lane group does not fetch 16 consecutive granules", name);
+            bool xor_rows = true;
+            for (int t = 0; t < P; ++t)
+                for (int q = 0; q < R / G; ++q)
+                    xor_rows = xor_rows && SW_IN::phys(PL::elem(0, t, q * G)) ==
+                                               (SW_IN::phys(PL::elem(0, t, 0)) ^ (q * G));
+            CHECK(xor_rows, "%s DMA: phase-0 row granules are not phys(row) ^ q", name);
+            std::printf("    LDS-DMA input staging: map verified\n");
+        }
+    }
+
     // 3. bank conflicts of every LDS instruction of the kernel, per wavefront (64 lanes)
     {
         ConflictReport rep;
