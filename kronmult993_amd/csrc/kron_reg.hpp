@@ -229,17 +229,44 @@ struct Lds
         return kVec && PL::P % 64 == 0 && (m & 0x000f000f000f000full) == 0;
     }
 
+    // When the map's parities see no granule bit at or above P, instruction c fetches granule c*P +
+    // phys(lane granule): one byte offset per thread (dma_offset0) plus a constant per instruction.
+    static constexpr bool dma_strided()
+    {
+        constexpr unsigned long long m = kronplan::swizzle_masks(PL::kD, PL::kN, PL::kRho, static_cast<int>(sizeof(T)), 0);
+        constexpr unsigned long long lo = static_cast<unsigned long long>(PL::P - 1);
+        return (m & ~(lo | (lo << 16) | (lo << 32) | (lo << 48))) == 0;
+    }
+
+    static __device__ __forceinline__ unsigned dma_offset0(int t)
+    {
+        return static_cast<unsigned>(SW_IN::phys(t * G) * sizeof(T));
+    }
+
     static __device__ __forceinline__ void dma_input(T *lds, T const *src, int t)
     {
         static_assert(dma_ok(), "the input map must be an involution on whole wavefront blocks");
         int const wbase = __builtin_amdgcn_readfirstlane(t & ~63);
         int const lane = t & 63;
+        // the thread index is made opaque here, so the offsets are formed at the loads (a few VALU
+        // instructions per entry) rather than kept in registers, or spilled, across the entry loop
+        int tt = t;
+        asm volatile("" : "+v"(tt));
+        unsigned const o = dma_offset0(tt);
+        auto base = (__attribute__((address_space(1))) char const *)src;
 #pragma unroll
         for (int c = 0; c < PL::R / G; ++c)
         {
-            int const p = c * PL::P + wbase + lane;  // physical granule written by this lane
-            int const q = SW_IN::phys(p * G) / G;    // logical granule that belongs there
-            auto g = (__attribute__((address_space(1))) void *)(src + q * G);
+            unsigned oc;
+            if constexpr (dma_strided())
+                oc = o + static_cast<unsigned>(c * PL::P * G * sizeof(T));
+            else
+            {
+                int const p = c * PL::P + wbase + lane;  // physical granule written by this lane
+                int const q = SW_IN::phys(p * G) / G;    // logical granule that belongs there
+                oc = static_cast<unsigned>(q * G * sizeof(T));
+            }
+            auto g = (__attribute__((address_space(1))) void *)(base + oc);
             auto l = (__attribute__((address_space(3))) void *)(lds + (c * PL::P + wbase) * G);
             __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
         }
