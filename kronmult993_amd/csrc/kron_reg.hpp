@@ -279,12 +279,14 @@ struct Lds
     static __device__ __forceinline__ void read0_dma(T const *lds, T (&v)[PL::R], int t)
     {
         static_assert(dma_ok() && PL::R / G <= 16, "granule index must stay below the mapped bits");
-        int b = SW_IN::phys(PL::elem(0, t, 0));
+        // byte offsets: granule q of the row is one XOR away from granule 0
+        unsigned b = static_cast<unsigned>(SW_IN::phys(PL::elem(0, t, 0)) * sizeof(T));
         asm volatile("" : "+v"(b));
+        auto base = reinterpret_cast<char const *>(lds);
 #pragma unroll
         for (int q = 0; q < PL::R / G; ++q)
         {
-            V g = *reinterpret_cast<V const *>(lds + (b ^ (q * G)));
+            V g = *reinterpret_cast<V const *>(base + (b ^ static_cast<unsigned>(q * 16)));
 #pragma unroll
             for (int i = 0; i < G; ++i) v[q * G + i] = g[i];
         }
