@@ -26,7 +26,7 @@
 // found by search under the measured bank model (conflict-free for all four access patterns).
 //
 // Measured on MI355X (8192-entry d=6 n=4 batch, FP64): 67 us per launch at 3 workgroups per CU against
-// 53 us for the register-resident sweep. The FP64 matrix rate of gfx950 equals its vector rate (one shared
+// 53 us for the register-resident sweep with register staging (48 us with LDS-DMA staging). The FP64 matrix rate of gfx950 equals its vector rate (one shared
 // FP64 datapath), so moving the FMAs to V_MFMA gains no throughput, and the permlane transposes and the
 // 3-waves-per-SIMD register budget (a few spilled values) cost more than the LDS exchange they replace.
 // The kernel is kept as an opt-in launch family (and an autotuner candidate) for parts whose FP64 matrix

@@ -257,7 +257,8 @@ hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T 
 
 // ---- matrix-core family (d = 6, n = 4, FP64; opt-in) -----------------------------------------------------
 // Same run structure as the register family. Not the default: on MI355X the FP64 matrix rate equals the
-// vector rate, and this kernel measured 67 us against the register sweep's 53 us on the d=6 n=4 batch.
+// vector rate, and this kernel measured 67 us against the register sweep's 53 us (48 us with LDS-DMA
+// staging) on the d=6 n=4 batch.
 hipError_t mfma_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, double const *const *mats,
                        double const *const *in, double *const *out, hipStream_t stream, bool run)
 {
