@@ -5,6 +5,8 @@ from functools import reduce
 
 import numpy as np
 
+from kronmult993_amd import workloads as W
+
 
 def normwise(y: np.ndarray, ref: np.ndarray) -> float:
     """max|y - ref| / max|ref| (the normwise relative error used throughout)."""
@@ -120,3 +122,14 @@ def terms_reference(b, fused: bool) -> np.ndarray:
             for t in range(terms):
                 out[o] = out[o] + y[t, k]
     return out.reshape(-1)
+
+
+def shift_odd_inputs(b):
+    """Each entry's input copied into its own slot of size+1 elements, odd entries shifted by one element:
+    the input pointers alternate between 16-byte and element alignment along every run of entries."""
+    size = b.size
+    pool = np.zeros(b.nb * (size + 1), dtype=b.dtype)
+    in_off = np.arange(b.nb, dtype=np.int64) * (size + 1) + (np.arange(b.nb) % 2)
+    for k in range(b.nb):
+        pool[in_off[k]: in_off[k] + size] = b.in_pool[b.in_off[k]: b.in_off[k] + size]
+    return W.HostBatch(**{**b.__dict__, "in_pool": pool, "in_off": in_off})

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import accumulation_bound, normwise
+from helpers import accumulation_bound, normwise, shift_odd_inputs
 from kronmult993_amd import api, workloads as W
 from kronmult993_amd.device import DeviceBatch
 
@@ -134,6 +134,23 @@ def test_register_plans_irregular(gpu, d, n, dtype):
     for k in range(b.nb):
         pool[in_off[k]: in_off[k] + size] = b.in_pool[b.in_off[k]: b.in_off[k] + size]
     b = W.HostBatch(**{**b.__dict__, "in_pool": pool, "in_off": in_off})
+    _check_aliased(b, _run(b, gpu))
+    u = _unaliased(b)
+    _check_bitexact(u, _run(u, gpu))
+
+
+@pytest.mark.parametrize("case", ["d6n4_runs", "d5n4_scattered"])
+def test_register_dma_staging_mixed_alignment(gpu, case):
+    """The FP64 d=5, 6 n=4 sweeps load inputs straight into LDS when the pointer is 16-byte aligned and
+    through registers otherwise: runs of two or more entries per workgroup that switch between the two
+    paths from one entry to the next, with same-output runs (d=6) and scattered outputs (d=5)."""
+    if case == "d6n4_runs":
+        b = W.c2(rows=32, cols=64)  # 2048 entries, runs of 64 entries per output
+    else:
+        b = W.random_case(91, 5, 4, 4500, stride=6, distinct=300)
+    info = api.launch_info(b.d, b.n, 8, b.nb)
+    assert info["family"] == api.FAMILY_REGISTER and info["entries_per_workgroup"] >= 2, info
+    b = shift_odd_inputs(b)
     _check_aliased(b, _run(b, gpu))
     u = _unaliased(b)
     _check_bitexact(u, _run(u, gpu))

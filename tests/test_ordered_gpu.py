@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import grouped_by_output, normwise, ordered_reference
+from helpers import grouped_by_output, normwise, ordered_reference, shift_odd_inputs
 from kronmult993_amd import api, workloads as W
 from kronmult993_amd.device import DeviceBatch
 
@@ -52,6 +52,13 @@ def test_ordered_c2_register_family(gpu, dtype):
 def test_ordered_register_range_lengths(gpu, epw):
     """Ranges shorter than, misaligned with and longer than the 64-entry output segments."""
     _check(W.c2(rows=5, cols=64), gpu, tuning={"entries_per_workgroup": epw})
+
+
+@pytest.mark.parametrize("epw", [3, 64])
+def test_ordered_register_mixed_alignment(gpu, epw):
+    """FP64 d=6 n=4 with input pointers alternating between 16-byte and element alignment inside every
+    range: the LDS-DMA input staging and its register fallback switch from one entry to the next."""
+    _check(shift_odd_inputs(W.c2(rows=5, cols=64)), gpu, tuning={"entries_per_workgroup": epw})
 
 
 @pytest.mark.parametrize("d,n,nb,dtype", [(3, 4, 300, np.float64), (3, 4, 300, np.float32), (5, 3, 90, np.float64),
