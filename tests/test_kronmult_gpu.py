@@ -156,6 +156,35 @@ def test_register_dma_staging_mixed_alignment(gpu, case):
     _check_bitexact(u, _run(u, gpu))
 
 
+@pytest.mark.parametrize("aliased", [False, True])
+def test_register_graph_replays(gpu, aliased):
+    """The stream-ordered call captured in a graph (FP64 d=6 n=4: LDS-DMA input staging, run sums, atomics),
+    replayed several times with eager output resets in between: unaliased outputs equal the oracle bit for
+    bit on every replay, aliased ones stay within the accumulation bound."""
+    import torch
+    b = shift_odd_inputs(W.c2(rows=6, cols=64))
+    if not aliased:
+        b = _unaliased(b)
+    print(f"[case] graph {b.name} nb={b.nb}", flush=True)
+    db = DeviceBatch(b, device=str(gpu))
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        db.reset_output()
+        s.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            db.run(sync=False, stream=s)
+    for replay in range(3):
+        db.reset_output()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        if aliased:
+            _check_aliased(b, db.output())
+        else:
+            _check_bitexact(b, db.output())
+
+
 @pytest.mark.parametrize("d,n,nb", [(3, 21, 6), (4, 12, 4)])
 def test_large_vectors_use_workspace(gpu, d, n, nb):
     """n^d * 8 bytes above the on-chip size, n > 8: the fiber sweep runs from the caller's workspace."""
