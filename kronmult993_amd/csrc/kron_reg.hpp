@@ -377,13 +377,14 @@ struct NoHook
 
 // Wave priority raised over a phase's contraction (S_SETPRIO 1, back to 0 before the LDS exchange): the
 // arbiter then favours waves that are in their FMA chains over waves that are between barriers. Measured
-// in the one-term sweep of the d=6 n=4 plan (FP64 47.2 against 48.0 us, FP32 30.9 against 33.5 us per
-// launch). The d=4 and d=5 n=4 plans measured no gain (d=5: 53.5 against 53.1 us), and the several-term
-// sweep lost (128.6 against 120.5 us per step), so those keep the default priority.
-template<typename PL, bool PRIO>
+// in the one-term sweep (per launch): d=6 n=4 FP64 47.2 against 48.0 us, FP32 30.9 against 33.5 us;
+// d=4 n=8 FP32 149 against 186 us; d=3 n=8 FP32 9.4 against 9.6 us; the other FP32 plans unchanged.
+// FP64 d=5 n=4 (53.5 against 53.1 us), FP64 d=4 n=8 (274 against 271 us) and the several-term sweep
+// (128.6 against 120.5 us per step) lost, so those keep the default priority.
+template<typename T, typename PL, bool PRIO>
 constexpr bool raise_priority_in_contraction()
 {
-    return PRIO && PL::kD == 6 && PL::kN == 4;
+    return PRIO && ((PL::kD == 6 && PL::kN == 4) || sizeof(T) == 4);
 }
 
 // the phases from PH on; after_last_read runs once the last phase's values are in registers (before its
@@ -393,9 +394,9 @@ __device__ __forceinline__ void run_phases(T *lds, T (&v)[PL::R], T const *const
                                            T const *coef, int t, F const &after_last_read = F())
 {
     if constexpr (PH + 1 == PL::NPHASE) after_last_read();
-    if constexpr (raise_priority_in_contraction<PL, PRIO>()) __builtin_amdgcn_s_setprio(1);
+    if constexpr (raise_priority_in_contraction<T, PL, PRIO>()) __builtin_amdgcn_s_setprio(1);
     contract_phase<T, PL, PH, PL::c_hi(PH) - 1>(v, A_list, stride, coef);
-    if constexpr (raise_priority_in_contraction<PL, PRIO>()) __builtin_amdgcn_s_setprio(0);
+    if constexpr (raise_priority_in_contraction<T, PL, PRIO>()) __builtin_amdgcn_s_setprio(0);
     if constexpr (PH + 1 < PL::NPHASE)
     {
         lds_barrier();  // every thread has read phase PH
