@@ -3,7 +3,7 @@
 // device alike.
 //
 // The n^d vector of one batch entry is a tensor with d modes of size n; mode 0 is the slowest index
-// (canonical element e = sum_t k_t n^(d-1-t)), as in the reference (tests/utils/kronmult_naive.h:99-101).
+// (canonical element e = sum_t k_t n^(d-1-t)), as in the reference (tests/utils/kronmult_naive.h:60-61).
 // A workgroup of P threads holds the whole tensor, R = n^RHO values per thread. The sweep runs in phases;
 // in phase p a thread's registers hold RHO consecutive modes [lo(p), lo(p)+RHO) and the remaining modes
 // index the thread. Phase p contracts the modes of its block that are due, from the fastest down, which

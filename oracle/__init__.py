@@ -2,8 +2,8 @@
 
 Python bindings over two CPU libraries built by oracle/Makefile:
   * liboracle.so: the project's C restatement of the reference arithmetic (kronmult_oracle.c), which
-    follows kronmult_omp/kronmult.hpp:64-136 and kronmult_omp/linear_algebra.hpp:214-257 (sweep) and
-    tests/utils/kronmult_naive.h:70-160 (dense oracle), plus an adjoint and an extended-precision sweep;
+    follows kronmult_omp/kronmult.hpp:32-104 and kronmult_omp/linear_algebra.hpp:78-121 (sweep) and
+    tests/utils/kronmult_naive.h:31-121 (dense oracle), plus an adjoint and an extended-precision sweep;
   * _ref/libkronmult_ref.so: the reference's own kronmult_omp header and dense test oracle compiled from
     their sources (present only where it was built).
 
@@ -25,9 +25,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_DIR = os.path.join(HERE, "_ref")
 # reference builds: "shipped" = the reference's own flags; "fused" / "unfused" = the two arithmetic
-# readings of `dotprod += x*a` (see oracle/Makefile)
+# readings of `dotprod += x*a`; "avx512" = the reference's flags for x86-64-v4 (see oracle/Makefile)
 REF_VARIANTS = {"shipped": "libkronmult_ref.so", "fused": "libkronmult_ref_fused.so",
-                "unfused": "libkronmult_ref_unfused.so"}
+                "unfused": "libkronmult_ref_unfused.so", "avx512": "libkronmult_ref_avx512.so"}
 REF_SO = os.path.join(REF_DIR, REF_VARIANTS["shipped"])
 
 _lib = None
@@ -72,6 +72,15 @@ def lib() -> ctypes.CDLL:
 
 def have_ref(variant: str = "shipped") -> bool:
     return os.path.exists(os.path.join(REF_DIR, REF_VARIANTS[variant]))
+
+
+def host_has_avx512() -> bool:
+    """True when this host's CPUs report AVX-512F (the avx512 build's baseline instruction set)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            return any(line.startswith("flags") and " avx512f" in line for line in f)
+    except OSError:
+        return False
 
 
 def ref_lib(variant: str = "shipped") -> ctypes.CDLL:

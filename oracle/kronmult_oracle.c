@@ -7,7 +7,7 @@
  * can be checked against it. Only the test suite, the smoke check and the CPU-baseline leg of the
  * benchmark load it; the GPU library never calls into it.
  *
- * Conventions (reference: kronmult_gpu/kronmult.cuh:12-27, tests/utils/kronmult_naive.h:86-106):
+ * Conventions (reference: kronmult_gpu/kronmult.cuh:12-27, tests/utils/kronmult_naive.h:47-67):
  *   - factor (r, c) of an n x n matrix with leading dimension `stride` sits at A[r + c*stride];
  *   - A_0 owns the slowest index of the n^d vector, A_{d-1} the fastest;
  *   - pointer arrays are entry-major: factor t of entry k is mats[k*d + t].

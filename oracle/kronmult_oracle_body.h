@@ -13,14 +13,14 @@
 /*
  * One batch entry of the backward sweep ("variant of the backward version of Algorithm 993").
  *
- * Restates kronmult_omp/kronmult.hpp:64-84 (loop over factors i = d-1 .. 0, ping-pong between `x`
- * and `w`, std::swap after each step) with the no-BLAS step of kronmult_omp/linear_algebra.hpp:238-257:
+ * Restates kronmult_omp/kronmult.hpp:32-60 (loop over factors i = d-1 .. 0, ping-pong between `x`
+ * and `w`, std::swap after each step) with the no-BLAS step of kronmult_omp/linear_algebra.hpp:102-121:
  *     W[colX + ncol*rowM] = sum_{k=0..n-1} X[k + n*colX] * A[rowM + k*stride]
  * where the k-sum starts from 0 and runs k ascending. `dotprod += X[...] * M_transposed[...]`
- * (linear_algebra.hpp:252) has two readings that compilers produce: `fused` != 0 takes every term as one
+ * (linear_algebra.hpp:116) has two readings that compilers produce: `fused` != 0 takes every term as one
  * fused multiply-add, acc = fma(x, a, acc) (g++ -O3 with FMA and contraction, loop not vectorized);
  * `fused` == 0 rounds the product first, acc = acc + x*a (ISO semantics, g++ -ffp-contract=off).
- * The transpose of linear_algebra.hpp:214-224 only changes where the coefficient is read from, so the
+ * The transpose of linear_algebra.hpp:78-88 only changes where the coefficient is read from, so the
  * coefficient A[rowM + k*stride] is read directly.
  *
  * `trans` != 0 reads A^T instead (coefficient A[k + rowM*stride]); this gives kron(A^T)·x for the
@@ -74,7 +74,7 @@ static ORC_T *ORC_NAME(orc_sweep_)(int d, int n, ORC_T const *const *mats, int s
 /*
  * output[k] += kron(A_{k,0}, ..., A_{k,d-1}) · input[k], entries processed in index order and the
  * accumulation done with plain sequential adds: this is kronmult_omp's kronmult_batched
- * (kronmult.hpp:109-136, accumulate loop kronmult.hpp:86-91) run on one OpenMP thread, whose `omp for`
+ * (kronmult.hpp:77-104, accumulate loop kronmult.hpp:54-59) run on one OpenMP thread, whose `omp for`
  * then visits entries in order. Unlike the reference, inputs are not clobbered (private copy).
  * Returns 0, or -1 on invalid arguments / allocation failure.
  */
@@ -107,7 +107,7 @@ int ORC_NAME(oracle_kronmult_batched_)(int d, int n, ORC_T const *const *mats_ba
 
 /*
  * Same arithmetic, OpenMP-parallel over entries with `omp atomic` accumulation, the threading scheme of
- * kronmult.hpp:117-135 and kronmult.hpp:86-91 (per-thread scratch, `omp for` over entries).
+ * kronmult.hpp:93-102 and kronmult.hpp:86-90 (per-thread scratch, `omp for` over entries).
  * Accumulation order into shared outputs depends on scheduling.
  */
 int ORC_NAME(oracle_kronmult_batched_omp_)(int d, int n, ORC_T const *const *mats_batched, int stride,
@@ -147,10 +147,10 @@ int ORC_NAME(oracle_kronmult_batched_omp_)(int d, int n, ORC_T const *const *mat
 }
 
 /*
- * Dense-Kronecker oracle. Restates tests/utils/kronmult_naive.h:112-160: the explicit product
- * K = A_0 ⊗ A_1 ⊗ ... ⊗ A_{d-1} is formed left to right with kronecker_product (kronmult_naive.h:86-106,
+ * Dense-Kronecker oracle. Restates tests/utils/kronmult_naive.h:73-121: the explicit product
+ * K = A_0 ⊗ A_1 ⊗ ... ⊗ A_{d-1} is formed left to right with kronecker_product (kronmult_naive.h:47-67,
  * row_out = row1*size2 + row2, i.e. A_0 holds the slowest index), then output += K·input with the
- * term-by-term accumulation of matrix_vector_product (kronmult_naive.h:70-80), each term fused or not
+ * term-by-term accumulation of matrix_vector_product (kronmult_naive.h:31-41), each term fused or not
  * as selected by `fused` (same two readings as the sweep).
  * O(n^(2d)) memory: the caller keeps n^d small (<= 4096 here).
  */
@@ -211,7 +211,7 @@ int ORC_NAME(oracle_dense_kronmult_batched_)(int d, int n, ORC_T const *const *m
 
 /*
  * Adjoint of output[k] += kron(A_{k,*}) · input[k] with respect to a cotangent g[k] = grad_output[k]
- * (no reference counterpart: the reference exports only the forward product, kronmult.hpp:109-136).
+ * (no reference counterpart: the reference exports only the forward product, kronmult.hpp:77-104).
  *
  *   grad_input[k]       += kron(A_{k,0}^T, ..., A_{k,d-1}^T) · g[k]        (the sweep on transposed factors)
  *   grad_mats[k*d+t][r + c*stride] += sum_m  G_(t)[r, m] · Z_(t)[c, m]

@@ -9,15 +9,15 @@
 #include <cstdlib>
 #include <omp.h>
 
-#include "kronmult.hpp"               // kronmult_omp/kronmult.hpp: kronmult_batched<T> (kronmult.hpp:109-136)
-#include "utils/kronmult_naive.h"     // tests/utils/kronmult_naive.h: kronmult_batched_naive<T> (148-160)
+#include "kronmult.hpp"               // kronmult_omp/kronmult.hpp: kronmult_batched<T> (kronmult.hpp:77-104)
+#include "utils/kronmult_naive.h"     // tests/utils/kronmult_naive.h: kronmult_batched_naive<T> (108-121)
 
 namespace
 {
 /*
  * Allocator handed to kronmult_naive. The reference asks for `size_kron * matrix_size` elements for the
- * next Kronecker matrix (kronmult_naive.h:125) but kronecker_product writes a square matrix of that
- * dimension (kronmult_naive.h:99-101), i.e. count^2 elements: a heap overflow for d >= 2 that the
+ * next Kronecker matrix (kronmult_naive.h:86) but kronecker_product writes a square matrix of that
+ * dimension (kronmult_naive.h:60-61), i.e. count^2 elements: a heap overflow for d >= 2 that the
  * reference's managed-memory GPU harness hides behind page-granular allocations. Allocating count^2
  * elements lets the reference code run unchanged.
  */

@@ -1,9 +1,9 @@
 // kronmult_bench_gpu.cpp -- the reference's GPU benchmark (tests/kronmult_bench_gpu.cpp) rewritten for
 // libkronmult_hip.so.
 //
-// Same five ASGarD-like cases (toy, small, medium, large, realistic; kronmult_bench_gpu.cpp:224-228) with
+// Same five ASGarD-like cases (toy, small, medium, large, realistic; kronmult_bench_gpu.cpp:68-72) with
 // the reference's stride 67 and 5 distinct outputs. Differences: the reference times one synchronous call
-// with a millisecond wall clock (kronmult_bench_gpu.cpp:194-200); here each case reports the wall time of
+// with a millisecond wall clock (kronmult_bench_gpu.cpp:38-44); here each case reports the wall time of
 // the synchronous drop-in call and the median kernel time over repeated stream-ordered launches measured
 // with HIP events, as GFLOP/s and contract GB/s (per entry 2*d*n^(d+1)+n^d flops and
 // 8*(d*n^2 + 3*n^d) + 8*(d+2) bytes). Data are seeded (the reference leaves them uninitialised).

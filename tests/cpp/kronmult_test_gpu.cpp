@@ -2,7 +2,7 @@
 // libkronmult_hip.so.
 //
 // Same cases as the reference (toy: degree 4, dimension 1, level 2; small: degree 4, dimension 2, level 4;
-// tests/kronmult_test_gpu.cpp:148-149) plus medium (degree 6, dimension 3, level 6) and the d=6 n=4 shape,
+// tests/kronmult_test_gpu.cpp:74-75) plus medium (degree 6, dimension 3, level 6) and the d=6 n=4 shape,
 // same stride 67 and 5 distinct outputs, compared with the project's CPU oracle (oracle/liboracle.so,
 // the restatement of kronmult_omp) instead of the dense product. Differences from the reference: seeded
 // data, every output element compared, non-zero exit status on failure.
@@ -20,7 +20,7 @@ using harness::DeviceBatch;
 // normwise relative error max|gpu - ref| / max|ref| over every element of every output
 static double run_case(int degree, int dimension, int level, char const *name, int distinct = 5)
 {
-    int const stride = 67;  // the reference's padded leading dimension (kronmult_test_gpu.cpp:95)
+    int const stride = 67;  // the reference's padded leading dimension (kronmult_test_gpu.cpp:21)
     int const nb = harness::compute_batch_size(degree, dimension, level, distinct);
     DeviceBatch<double> b(dimension, degree, stride, nb, distinct, 0, true, false);
     std::printf("%s: degree %d dimension %d level %d -> nb_batch %d, n^d %ld\n", name, degree, dimension, level, nb,

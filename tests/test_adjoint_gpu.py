@@ -200,3 +200,81 @@ def test_autograd_gradcheck(gpu):
     # differ in the last bits, hence a nondeterminism tolerance
     assert torch.autograd.gradcheck(lambda m, x, o: kronmult(m, x, o, layout), (mat, inp, out0),
                                     eps=1e-6, atol=1e-7, rtol=1e-6, nondet_tol=1e-13)
+
+
+# ---- the differentiation workload's own checks, on 64 entries of its shape -----------------------------
+
+def _c5_subset():
+    """64 entries of the d=3 n=8 differentiation batch's shape: 8 shared factor triples (8 entries each,
+    entry k uses triple k mod 8), outputs k/4, inputs U[-1,1], cotangent N(0,1)."""
+    return W.c5(nb=64, triples=8)
+
+
+def _dense_gradients(b):
+    """The dense-Kronecker gradient of L = sum_o <g_o, y_o>: d(input_k) = K_k^T g_o(k) with K_k the explicit
+    n^d x n^d Kronecker matrix, and d(A_t)[r, c] = sum over the entries of K_k's elements whose mode-t indices
+    are (r, c), of (g x^T)[i, j] times the other factors' elements: the gradient of <g, K x> with respect to
+    one factor of the product K = A_0 (x) A_1 (x) A_2, read off the dense outer product g x^T."""
+    n, d, size = b.n, b.d, b.size
+    assert d == 3
+    ri, rm = np.zeros_like(b.in_pool), np.zeros_like(b.mat_pool)
+    for k in range(b.nb):
+        A = factors(b, k)
+        gk = b.grad_out_pool[int(b.out_off[k]): int(b.out_off[k]) + size]
+        xk = b.in_pool[int(b.in_off[k]): int(b.in_off[k]) + size]
+        ri[int(b.in_off[k]): int(b.in_off[k]) + size] += kron_dense(A).T @ gk
+        GX = np.outer(gk, xk).reshape(n, n, n, n, n, n)  # (i0, i1, i2, j0, j1, j2)
+        grads = (np.einsum("abcdef,be,cf->ad", GX, A[1], A[2]),
+                 np.einsum("abcdef,ad,cf->be", GX, A[0], A[2]),
+                 np.einsum("abcdef,ad,be->cf", GX, A[0], A[1]))
+        for t in range(d):
+            off = int(b.mat_off[k * d + t])
+            for c in range(n):
+                rm[off + c * b.stride: off + c * b.stride + n] += grads[t][:, c]
+    return ri, rm
+
+
+@pytest.mark.parametrize("intermediates", ["recompute", "store"])
+def test_adjoint_c5_subset_dense_gradient(gpu, intermediates):
+    b = _c5_subset()
+    gi, gm = _gpu_adjoint(b, gpu, intermediates)
+    ri, rm = _dense_gradients(b)
+    assert normwise(gi, ri) <= 1e-13
+    assert normwise(gm, rm) <= 1e-13
+    # the dense gradient itself against the oracle's adjoint (pinned on the CPU)
+    oi, om = oracle.adjoint(b)
+    assert normwise(oi, ri) <= 1e-13 and normwise(om, rm) <= 1e-13
+
+
+def test_adjoint_c5_subset_central_differences(gpu):
+    """Central differences (h = 1e-6) of the GPU forward, through the autograd Function, at factor and input
+    elements drawn from every shared triple and many entries."""
+    import torch
+    b = _c5_subset()
+    kronmult, layout, mat, inp, out0, g = _autograd_setup(b, gpu)
+    out = kronmult(mat, inp, out0, layout)
+    (out * g).sum().backward()
+    gm, gi = mat.grad.cpu().numpy(), inp.grad.cpu().numpy()
+
+    def loss(m, x):
+        with torch.no_grad():
+            return float((kronmult(m, x, out0.detach(), layout) * g).sum())
+
+    h = 1e-6
+    rng = np.random.default_rng(5)
+    with torch.no_grad():
+        L_abs = float((kronmult(mat.detach().abs(), inp.detach().abs(), out0.detach().abs(), layout) * g.abs()).sum())
+    idx_m = rng.choice(b.mat_pool.size, 24, replace=False)
+    used_i = np.unique((b.in_off[:, None] + np.arange(b.size)[None, :]).reshape(-1))
+    idx_i = rng.choice(used_i, 24, replace=False)
+    for which, idxs in (("mat", idx_m), ("in", idx_i)):
+        for i in idxs:
+            m, x = mat.detach().clone(), inp.detach().clone()
+            t = m if which == "mat" else x
+            t[i] += h
+            lp = loss(m, x)
+            t[i] -= 2 * h
+            lm = loss(m, x)
+            fd = (lp - lm) / (2 * h)
+            ad = (gm if which == "mat" else gi)[i]
+            assert abs(fd - ad) <= 1e-8 * max(1.0, L_abs), f"{which}[{i}]: fd {fd!r} vs adjoint {ad!r}"

@@ -46,9 +46,12 @@ def test_gpu_against_reference_fixtures(gpu, path):
     db.run()
     got = db.output()
     ref = z["out_fused"]
+    tol = 1e-13 if b.dtype == np.float64 else 1e-5
     if np.bincount(b.out_index, minlength=b.out_count).max() == 1:
         assert np.array_equal(got, ref)
     else:
-        tol = 1e-13 if b.dtype == np.float64 else 1e-5
         assert normwise(got, ref) <= tol
+    # against the reference as built with its own flags (g++ -O3: vectorised k-loop with rounded products,
+    # scalar tail fused): the target bound of the drop-in, normwise
+    assert normwise(got, z["out_shipped"]) <= tol
     assert np.all(np.isfinite(got))

@@ -89,7 +89,8 @@ def test_c3_groups(gpu, dtype, group):
 
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_c4_range(gpu, dtype):
-    b = W.c4(nb=4096).astype(dtype)
+    """The precision workload at its full size (16384 entries, runs of 16 entries per output)."""
+    b = W.c4().astype(dtype)
     _check_aliased(b, _run(b, gpu))
     u = _unaliased(W.c4(nb=256).astype(dtype))
     _check_bitexact(u, _run(u, gpu))
@@ -103,7 +104,7 @@ def test_c5_forward(gpu, dtype):
 
 @pytest.mark.parametrize("deg,dim,lvl", [(4, 1, 2), (4, 2, 4), (6, 3, 6)])
 def test_reference_harness_cases(gpu, deg, dim, lvl):
-    """The reference's own test cases (tests/kronmult_test_gpu.cpp:148-149 and the bench's medium case):
+    """The reference's own test cases (tests/kronmult_test_gpu.cpp:74-75 and the bench's medium case):
     stride 67, N(0,1) data, non-zero outputs aliased by tests/utils/utils_cpu.h:79-83."""
     b = W.reference_case(deg, dim, lvl)
     _check_aliased(b, _run(b, gpu))

@@ -4,7 +4,7 @@ Each check is chosen so that a plausible mistake in the oracle (a factor transpo
 reversed, a dropped term, a wrong stride) fails at least one of them:
   * bit-exact agreement with the reference's own kronmult_omp and kronmult_naive, compiled from
     /root/reference (oracle/_ref), in both arithmetic readings of `dotprod += x*a`;
-  * np.kron brute force (A_0 the slowest index, tests/utils/kronmult_naive.h:99-101);
+  * np.kron brute force (A_0 the slowest index, tests/utils/kronmult_naive.h:60-61);
   * closed forms: identity, permutation, diagonal, rank-1 factors, d=1 GEMV, d=2 vec(B X A^T);
   * invariants: linearity in x, multilinearity in each factor, the mixed-product property, aliasing;
   * the adjoint against central finite differences and the dense Kronecker gradient.
@@ -273,3 +273,79 @@ def test_planted_reversed_factor_order_is_caught():
     moff = b.mat_off.reshape(b.nb, b.d)[:, ::-1].reshape(-1).copy()
     mutated = oracle.kronmult(W.HostBatch(**{**b.__dict__, "mat_off": moff}))
     assert normwise(mutated, numpy_kronmult(b)) > 1e-3
+
+
+# ---- the OpenMP restatement and the FP32 adjoint (both pinned against functions pinned above) ------------
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nthreads", [2, 4])
+def test_omp_oracle_unaliased_equals_sequential(dtype, nthreads):
+    """One contribution per output: the OpenMP restatement (kronmult.hpp:77-104 threading) must equal the
+    sequential restatement bit for bit, in both arithmetic readings, whatever the schedule."""
+    b = W.random_case(31, 4, 3, 96, distinct=96, dtype=dtype)
+    for fused in (True, False):
+        assert np.array_equal(oracle.kronmult_omp(b, nthreads, fused=fused), oracle.kronmult(b, fused=fused))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_omp_oracle_aliased_within_reordering_bound(dtype):
+    """Aliased outputs: only the order of the atomic additions may differ from the sequential sum."""
+    from helpers import accumulation_bound
+    b = W.random_case(32, 3, 4, 400, distinct=9, dtype=dtype)
+    seq = oracle.kronmult(b).astype(np.float64)
+    ulp = float(np.finfo(dtype).eps)
+    bound = accumulation_bound(b, ulp)
+    for nthreads in (3, 8):
+        got = oracle.kronmult_omp(b, nthreads).astype(np.float64)
+        assert np.all(np.abs(got - seq) <= bound), nthreads
+    # a planted mistake (one entry's contribution dropped) is far outside the bound
+    short = W.HostBatch(**{**b.__dict__, "mat_off": b.mat_off[:-b.d].copy(), "in_off": b.in_off[:-1].copy(),
+                           "out_index": b.out_index[:-1].copy()})
+    assert np.any(np.abs(oracle.kronmult_omp(short, 4).astype(np.float64) - seq) > bound)
+
+
+@needs_ref
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_omp_oracle_vs_reference_threads(dtype):
+    """The OpenMP restatement against the reference's own kronmult_omp run on the same thread count: bit for
+    bit with one contribution per output, within the reordering bound with aliasing."""
+    from helpers import accumulation_bound
+    u = W.random_case(33, 3, 5, 120, distinct=120, dtype=dtype)
+    assert np.array_equal(oracle.kronmult_omp(u, 4), oracle.ref_kronmult(u, nthreads=4, variant="fused"))
+    a = W.random_case(34, 3, 4, 300, distinct=7, dtype=dtype)
+    bound = accumulation_bound(a, float(np.finfo(dtype).eps))
+    diff = oracle.kronmult_omp(a, 4).astype(np.float64) - oracle.ref_kronmult(a, nthreads=4, variant="fused")
+    assert np.all(np.abs(diff) <= 2 * bound)
+
+
+@pytest.mark.parametrize("d,n,shared", [(3, 4, None), (3, 8, 4), (2, 5, 2)])
+def test_adjoint_f32_vs_f64(d, n, shared):
+    """The FP32 adjoint is the FP64 one (pinned above by finite differences and the dense gradient) up to
+    FP32 rounding: componentwise within c*u32 of the same adjoint on |A|, |x|, |g|."""
+    b = W.random_case(35 + d + n, d, n, 40, distinct=10, shared_factors=shared)
+    g = b.grad_out_pool
+    gi64, gm64 = oracle.adjoint(b, g)
+    b32 = b.astype(np.float32)
+    gi32, gm32 = oracle.adjoint(b32, g.astype(np.float32))
+    # magnitudes: the same adjoint on absolute values (each gradient entry is a sum of products)
+    absb = W.HostBatch(**{**b.__dict__, "mat_pool": np.abs(b.mat_pool), "in_pool": np.abs(b.in_pool)})
+    mi, mm = oracle.adjoint(absb, np.abs(g))
+    u32 = float(np.finfo(np.float32).eps)
+    c = 4 * (d + 2) * n  # rounding of the float32 inputs plus the depth of each sum
+    assert np.all(np.abs(gi32.astype(np.float64) - gi64) <= c * u32 * mi)
+    assert np.all(np.abs(gm32.astype(np.float64) - gm64) <= c * u32 * mm)
+    assert normwise(gi32.astype(np.float64), gi64) > 0.0  # really computed in float32
+    # a planted mistake in the FP32 run (the cotangent of one output negated) leaves the bound
+    g_bad = g.astype(np.float32).copy()
+    g_bad[: b.size] *= -1.0
+    bad_i, _ = oracle.adjoint(b32, g_bad)
+    assert np.any(np.abs(bad_i.astype(np.float64) - gi64) > c * u32 * mi)
+
+
+@pytest.mark.skipif(not (oracle.have_ref("avx512") and oracle.host_has_avx512()),
+                    reason="AVX-512 build of the reference absent or host without AVX-512F")
+@pytest.mark.parametrize("case", [0, 1, 6, 7])
+def test_reference_avx512_build_close_to_oracle(case):
+    """The benchmark's CPU baseline on AVX-512 hosts: the reference's flags for x86-64-v4."""
+    b = CASES[case]()
+    assert normwise(oracle.ref_kronmult(b, nthreads=1, variant="avx512"), oracle.kronmult(b)) < 64 * np.finfo(np.float64).eps
