@@ -74,16 +74,31 @@ def cpu_threads() -> int:
     return HOST_CPUS
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(hbs, runs: int):
     """The reference's kronmult_omp (built from its header, its own flags) on all host threads, one call per
     batch (a grouped workload is one call per group, back to back), full batches."""
     import oracle
     threads = cpu_threads()
     kind = "reference" if oracle.have_ref("shipped") else "port"
+    # the reference builds with -march=native (tests/CMakeLists.txt:5): on an AVX-512 host its x86-64-v4
+    # build is the nearest one made in the build container, else the x86-64-v3 build
+    variant = "avx512" if kind == "reference" and oracle.have_ref("avx512") and oracle.host_has_avx512() \
+        else "shipped"
     calls, resets = [], []
     for hb in hbs:
         if kind == "reference":
-            runner = oracle.RefRunner(hb, "shipped")
+            runner = oracle.RefRunner(hb, variant)
             calls.append(lambda r=runner: r.run(threads))
             resets.append(runner.reset)
         else:  # the restatement, same arithmetic and threading scheme
@@ -103,7 +118,11 @@ def cpu_baseline(hbs, runs: int):
         log(f"[cpu] run {r} {'(warm-up) ' if r == 0 else ''}{(t1 - t0) * 1e3:.1f} ms on {threads} threads")
     med = statistics.median(times)
     flops = sum(hb.flops() for hb in hbs)
-    placement = (f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} OMP_PLACES={os.environ.get('OMP_PLACES')} "
+    build = ({"avx512": "reference kronmult_omp, g++ -O3 -march=x86-64-v4 -fopenmp (AVX-512)",
+              "shipped": "reference kronmult_omp, g++ -O3 -march=x86-64-v3 -fopenmp (AVX2+FMA)"}[variant]
+             if kind == "reference" else "the oracle's OpenMP restatement")
+    placement = (f"{build}; {cpu_model()}; "
+                 f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} OMP_PLACES={os.environ.get('OMP_PLACES')} "
                  f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')}, {HOST_CPUS} CPUs in this process's "
                  f"affinity mask at start, {os.cpu_count()} online")
     names = "+".join(hb.name for hb in hbs)
@@ -128,6 +147,72 @@ def load_traffic(key: str, kernel: str):
     return entry.get("bytes_per_launch")
 
 
+def time_steps(step, stream, dev, steps: int, warmup: int):
+    """W untimed warm-up steps; an eager pass with HIP events around every launch on the launch stream (the
+    kernel duration); then the timed region: one replay of a graph holding K launches, bracketed by device
+    synchronisation. Returns (ms per step of the replay, per-launch eager ms, wall seconds of the replay)."""
+    import torch
+    for _ in range(warmup):
+        step(stream)
+    torch.cuda.synchronize(dev)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    for i in range(steps):
+        starts[i].record(stream)
+        step(stream)
+        ends[i].record(stream)
+    torch.cuda.synchronize(dev)
+    kernel_ms = [a.elapsed_time(b) for a, b in zip(starts, ends)]
+    graph = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream(device=dev)
+    cap.wait_stream(stream)
+    with torch.cuda.stream(cap):
+        with torch.cuda.graph(graph, stream=cap):
+            for _ in range(steps):
+                step(cap)
+    torch.cuda.synchronize(dev)
+    graph.replay()  # warm replay
+    torch.cuda.synchronize(dev)
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    graph.replay()
+    t_end.record(stream)
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - wall0
+    return t_start.elapsed_time(t_end) / steps, kernel_ms, wall
+
+
+def split_leg(db, stream, dev, steps: int, warmup: int, fp64_ms_per_step: float, flops: int):
+    """The split-precision half of the metric on the same batch: FP64 data through the FP32 matrix cores
+    (kronmult_batched_split_f64, error-free fixed-point splitting), timed like the FP64 step, with its error
+    against the FP64 kernel's result on one clean call (that kernel is pinned to the oracle by the tests:
+    within 1e-13 normwise)."""
+    import numpy as np
+    import torch
+    step = lambda s: db.run_split("fixed3", sync=False, stream=s)
+    ms, kernel_ms, _ = time_steps(step, stream, dev, steps, warmup)
+    db.reset_output()
+    db.run(sync=True)
+    ref = db.output() - db.hb.output_pool_init()
+    db.reset_output()
+    db.run_split("fixed3", sync=True)
+    got = db.output() - db.hb.output_pool_init()
+    torch.cuda.synchronize(dev)
+    err = np.abs(got - ref)
+    scale = float(np.max(np.abs(ref)))
+    k_avg = statistics.mean(kernel_ms)
+    log(f"[bench] split fixed3: {ms * 1e3:.2f} us/step (graph replay), kernel avg {k_avg * 1e3:.2f} us, "
+        f"max rel err {float(np.max(err)) / scale:.3e}")
+    return {"kernel": "split_fixed3", "ms_per_step": round(ms, 6), "value": round(flops / (ms * 1e-3) / 1e9, 2),
+            "unit": "GFLOP/s (FP64-equivalent)", "kernel_us": round(k_avg * 1e3, 3),
+            "max_rel_err": float(np.max(err)) / scale,
+            "rms_rel_err": float(np.sqrt(np.mean(err ** 2)) / np.sqrt(np.mean(ref ** 2))),
+            "error_reference": "this batch through kronmult_batched_f64 (FP64 kernel, within 1e-13 of the oracle)",
+            "speedup_vs_fp64": round(fp64_ms_per_step / ms, 4), "target_rel_err": 1e-10}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +228,8 @@ def main() -> int:
     ap.add_argument("--intermediates", default="auto", choices=["auto", "recompute", "store"],
                     help="adjoint: how the forward intermediates are obtained")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
+    ap.add_argument("--split", default="on", choices=["on", "off"],
+                    help="forward FP64 runs: also time the split-precision path on the same batch")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, copy a 256 MiB device buffer (reference kernel for byte counters)")
@@ -215,42 +302,8 @@ def main() -> int:
     log(f"[bench] {'+'.join(hb.name for hb in batches)} {args.dtype}: kernel={kernel} on "
         f"{torch.cuda.get_device_name(dev)}")
 
-    for _ in range(args.warmup):
-        step(stream)
-    torch.cuda.synchronize(dev)
-
-    # eager pass: HIP events around every launch on the launch stream -> kernel duration
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    for i in range(args.steps):
-        starts[i].record(stream)
-        step(stream)
-        ends[i].record(stream)
-    torch.cuda.synchronize(dev)
-    kernel_ms = [a.elapsed_time(b) for a, b in zip(starts, ends)]
+    ms_per_step, kernel_ms, wall = time_steps(step, stream, dev, args.steps, args.warmup)
     k_avg = statistics.mean(kernel_ms)
-
-    # timed region: one replay of a graph holding the K launches
-    graph = torch.cuda.CUDAGraph()
-    cap = torch.cuda.Stream(device=dev)
-    cap.wait_stream(stream)
-    with torch.cuda.stream(cap):
-        with torch.cuda.graph(graph, stream=cap):
-            for _ in range(args.steps):
-                step(cap)
-    torch.cuda.synchronize(dev)
-    graph.replay()  # warm replay
-    torch.cuda.synchronize(dev)
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    wall0 = time.perf_counter()
-    t_start.record(stream)
-    graph.replay()
-    t_end.record(stream)
-    torch.cuda.synchronize(dev)
-    wall = time.perf_counter() - wall0
-    total_ms = t_start.elapsed_time(t_end)
-    ms_per_step = total_ms / args.steps
     log(f"[bench] {args.steps} steps: {ms_per_step * 1e3:.2f} us/step (graph replay), kernel avg "
         f"{k_avg * 1e3:.2f} us (eager, min {min(kernel_ms) * 1e3:.2f}), wall {wall * 1e6 / args.steps:.2f} us/step")
 
@@ -264,13 +317,21 @@ def main() -> int:
         per_byte = s_ * (4 * hb.size + 3 * hb.d * hb.n * hb.n) + 8 * (2 + 2 * hb.d)
         flops, contract = hb.nb * per_flop, hb.nb * per_byte
     gflops = flops / (ms_per_step * 1e-3) / 1e9
-    achieved_gbs = contract / (k_avg * 1e-3) / 1e9
+    contract_gbs = contract / (k_avg * 1e-3) / 1e9
     peak_tf = FP64_PEAK_TFLOPS if args.dtype == "f64" else FP32_PEAK_TFLOPS
     achieved_tf = flops / (k_avg * 1e-3) / 1e12
     traffic_key = f"{args.config}_{args.dtype}" + ("" if args.op == "forward" else f"_{args.op}")
     traffic = load_traffic(traffic_key, kernel)
-    # binding roofline: the HBM time of the bytes that actually move (measured traffic when recorded, the
-    # per-entry contract bytes otherwise) against the compute time at peak
+    # the arithmetic runs on the vector FMA units (the adjoint's factor gradients partly on the FP64 matrix
+    # cores, same peak on MI355X); the split path's on the FP32 matrix cores
+    pipe = "fp32_mfma" if args.op == "split" else ("fp64_valu" if args.dtype == "f64" else "fp32_valu")
+    if args.op == "split":
+        peak_tf = FP32_PEAK_TFLOPS
+    # HBM: the bytes that actually move (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size)
+    # over the kernel duration; the per-entry contract bytes are reported beside it, not as a fraction
+    hbm_gbs = traffic / (k_avg * 1e-3) / 1e9 if traffic is not None else None
+    # binding roofline: the longer of the measured bytes at 8 TB/s (contract bytes when unmeasured) and the
+    # flops at the pipe's peak
     t_hbm = (traffic if traffic is not None else contract) / (HBM_PEAK_GBS * 1e9)
     t_cmp = flops / (peak_tf * 1e12)
     result = {
@@ -289,19 +350,24 @@ def main() -> int:
         "config": dict({"workload": f"{args.config}: {WORKLOADS[args.config]}", "kernel": kernel},
                        **extra_config),
         "roofline": None,  # the binding one of the two below
-        "roofline_hbm": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
-        "roofline_compute": {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": peak_tf,
+        "roofline_hbm": {"bound": "hbm", "achieved": None if hbm_gbs is None else round(hbm_gbs, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": None if hbm_gbs is None else round(hbm_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                         "note": "measured HBM bytes per launch over the kernel duration"},
+        "roofline_compute": {"bound": pipe, "achieved": round(achieved_tf, 3), "peak": peak_tf,
                              "unit": "TFLOP/s", "frac": round(achieved_tf / peak_tf, 4), "traffic": traffic,
-                             "note": ("FP64 vector and matrix peak are equal on MI355X (78.6 TFLOP/s); the sweep "
-                                      "runs on the vector FMA units" if args.dtype == "f64" else
-                                      "FP32 peak 157.3 TFLOP/s")},
+                             "note": ("FP64 vector peak of MI355X (the FP64 matrix peak is the same, 78.6 "
+                                      "TFLOP/s); the sweep runs on the vector FMA units" if pipe == "fp64_valu"
+                                      else "FP32 peak 157.3 TFLOP/s")},
+        "contract_gbs": round(contract_gbs, 1),
         "kernel_us": {"avg": round(k_avg * 1e3, 3), "min": round(min(kernel_ms) * 1e3, 3),
                       "max": round(max(kernel_ms) * 1e3, 3)},
         "contract_bytes_per_launch": contract,
         "flops_per_launch": flops,
     }
     result["roofline"] = result["roofline_compute"] if t_cmp > t_hbm else result["roofline_hbm"]
+    if args.op == "forward" and args.split == "on" and args.dtype == "f64" and len(dbs) == 1:
+        result["split"] = split_leg(dbs[0], stream, dev, args.steps, args.warmup, ms_per_step, flops)
     if args.cpu_baseline == "auto" and args.op == "forward":
         result["cpu_baseline"] = cpu_baseline(batches, args.cpu_runs)
     if args.calibrate_copy:

@@ -23,6 +23,9 @@
 #include <climits>
 #include <cstdint>
 #include <cstdio>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "kron_adjoint.hpp"
@@ -196,17 +199,29 @@ int device_cu_count()
     return v;
 }
 
-// Resident workgroups per CU of one kernel instantiation (cached when the launch takes no dynamic LDS:
-// then it depends only on the code object; otherwise queried per call, the LDS varying with the shape).
+// Resident workgroups per CU of one kernel (cached per device, kernel address and block size when the
+// launch takes no dynamic LDS: then it depends only on the code object; otherwise queried per call, the LDS
+// varying with the shape). Kernels of one plan family share a function type, so the key is the address.
 template<typename Kernel>
 int resident_per_cu(Kernel kernel, int block, size_t dyn_lds)
 {
-    static std::atomic<int> cache{0};
-    int v = dyn_lds == 0 ? cache.load(std::memory_order_relaxed) : 0;
-    if (v == 0)
+    static std::mutex lock;
+    static std::map<std::tuple<int, void const *, int>, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+    auto const key = std::make_tuple(dev, reinterpret_cast<void const *>(kernel), block);
+    if (dyn_lds == 0)
     {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, block, dyn_lds) != hipSuccess || v <= 0) v = 1;
-        if (dyn_lds == 0) cache.store(v, std::memory_order_relaxed);
+        std::lock_guard<std::mutex> guard(lock);
+        auto const it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    int v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, block, dyn_lds) != hipSuccess || v <= 0) v = 1;
+    if (dyn_lds == 0)
+    {
+        std::lock_guard<std::mutex> guard(lock);
+        cache[key] = v;
     }
     return v;
 }
