@@ -133,6 +133,26 @@ int kronmult_batched_ordered_f32_async(int matrix_count, int matrix_size, float 
                                        int matrix_stride, float **input_batched, float **output_batched,
                                        float **workspace_batched, int nb_batch, kronmult_tuning_t const *tuning,
                                        void *stream);
+
+/*
+ * Ordered accumulation that allocates nothing and takes no workspace. No counterpart in the reference; it
+ * is for callers that cannot allocate device memory or hand over a workspace: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. Same meaning and summation order as
+ * kronmult_batched_ordered_f64_async (stream-ordered, layout trusted), with the partial sums and the
+ * workspace pointer array placed in a library-owned device array of KRONMULT_NOALLOC_SCRATCH_BYTES on each
+ * GPU (created when the library's code is loaded on that GPU, held until the process ends; its address is
+ * looked up on the first call on each device). A call needs
+ *     round_up(8 * nb_batch, 256) + slots * n^d * 8 bytes,
+ * slots = nb_batch when the range length (kronmult_hip_ordered_launch_info) is 1, else 2 * ceil(nb_batch /
+ * range length); a larger batch returns hipErrorOutOfMemory before anything is launched. Nothing in the
+ * array needs clearing between calls. All calls on one GPU share the array: calls ordered on one stream are
+ * safe, calls that overlap in time on one GPU (several streams or host threads) race and give wrong results.
+ * Returns a hipError_t code. The d=6 n=4 operator batch (8192 entries, range length 8) needs 64 MiB.
+ */
+#define KRONMULT_NOALLOC_SCRATCH_BYTES (96u * 1024u * 1024u)
+int kronmult_batched_ordered_f64_noalloc(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                                         int matrix_stride, double **input_batched, double **output_batched,
+                                         int nb_batch, kronmult_tuning_t const *tuning, void *stream);
 /*
  * Several Kronecker terms per entry (no counterpart in the reference, which takes one term per call; the
  * ASGarD operator is a sum of such terms over the same element pairs):

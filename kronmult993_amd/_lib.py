@@ -62,6 +62,7 @@ SIGNATURES = {
     "kronmult_batched_ordered_f32": (_I, [_I, _I, _P, _I, _P, _P, _P, _I]),
     "kronmult_batched_ordered_f64_async": (_I, [_I, _I, _P, _I, _P, _P, _P, _I, ctypes.POINTER(Tuning), _P]),
     "kronmult_batched_ordered_f32_async": (_I, [_I, _I, _P, _I, _P, _P, _P, _I, ctypes.POINTER(Tuning), _P]),
+    "kronmult_batched_ordered_f64_noalloc": (_I, [_I, _I, _P, _I, _P, _P, _I, ctypes.POINTER(Tuning), _P]),
     "kronmult_hip_ordered_launch_info": (_I, [_I, _I, _I, _I, ctypes.POINTER(Tuning), ctypes.POINTER(LaunchInfo)]),
     "kronmult_batched_terms_f64": (_I, [_I, _I, _I, _P, _I, _P, _P, _P, _I]),
     "kronmult_batched_terms_f32": (_I, [_I, _I, _I, _P, _I, _P, _P, _P, _I]),

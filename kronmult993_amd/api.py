@@ -20,6 +20,8 @@ from ._lib import GroupedOptions, GroupF32, GroupF64, LaunchInfo, Tuning, lib
 
 HIP_SUCCESS = 0
 HIP_ERROR_INVALID_VALUE = 1
+HIP_ERROR_OUT_OF_MEMORY = 2
+NOALLOC_SCRATCH_BYTES = 96 * 1024 * 1024  # KRONMULT_NOALLOC_SCRATCH_BYTES
 
 
 class KronmultError(RuntimeError):
@@ -139,6 +141,20 @@ def kronmult_batched_ordered(matrix_count: int, matrix_size: int, matrix_list_ba
             _stream_handle(stream))
     if rc != HIP_SUCCESS:
         raise KronmultError(rc, "kronmult_batched_ordered")
+
+
+def kronmult_batched_ordered_noalloc(matrix_count: int, matrix_size: int, matrix_list_batched, matrix_stride: int,
+                                     input_batched, output_batched, nb_batch: int, *, tuning=None,
+                                     stream=None) -> None:
+    """Ordered accumulation (FP64) with no workspace and no allocation: the partial sums live in the library's
+    own device array (NOALLOC_SCRATCH_BYTES per GPU). Stream-ordered, layout trusted; raises KronmultError
+    with code HIP_ERROR_OUT_OF_MEMORY when the batch needs more than that array."""
+    tu = _tuning(tuning)
+    rc = lib().kronmult_batched_ordered_f64_noalloc(
+        matrix_count, matrix_size, _addr(matrix_list_batched), matrix_stride, _addr(input_batched),
+        _addr(output_batched), nb_batch, ctypes.byref(tu) if tu else None, _stream_handle(stream))
+    if rc != HIP_SUCCESS:
+        raise KronmultError(rc, "kronmult_batched_ordered_f64_noalloc")
 
 
 def ordered_launch_info(matrix_count: int, matrix_size: int, element_size: int, nb_batch: int,

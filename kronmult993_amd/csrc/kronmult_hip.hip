@@ -1159,6 +1159,81 @@ int kronmult_ordered_sync(int d, int n, T const *const *mats, int stride, T **in
     return static_cast<int>(hipDeviceSynchronize());
 }
 
+// ---- ordered accumulation without a workspace or an allocation ---------------------------------------------
+// The partial sums of an ordered call, and the per-entry workspace pointer array the kernels read, live in one
+// library-owned device array per GPU. Layout of a call: nb pointers (rounded up to 256 bytes), then slots of
+// n^d values. A range of epw entries owns two slots: r for the piece that starts at the range's first entry,
+// ranges + r for the piece that starts inside the range and continues past its end (the only other piece of
+// a range that is not a whole segment). One-entry ranges (epw == 1) take one slot each, slot = entry.
+static __device__ __align__(256) unsigned char scratch[KRONMULT_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on the current device, looked up once per device with that device current:
+// each GPU holds its own copy of the array. Host code passes this address, never the array's name.
+hipError_t scratch_address(void **ptr)
+{
+    static std::mutex lock;
+    static void *addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(lock);
+    if (addr[dev] == nullptr)
+    {
+        void *p = nullptr;
+        err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess) return err;
+        if (p == nullptr) return hipErrorInvalidSymbol;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    return hipSuccess;
+}
+
+template<typename T>
+__global__ void __launch_bounds__(256)
+    scratch_workspace_kernel(int nb, int epw, long ranges, long size, T **wsp, T *slots)
+{
+    long const k = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    long const r = k / epw;
+    wsp[k] = slots + ((k % epw == 0) ? r : ranges + r) * size;
+}
+
+template<typename T>
+hipError_t kronmult_ordered_noalloc(int d, int n, T const *const *mats, int stride, T **in, T **out, int nb,
+                                    kronmult_tuning_t const *tuning, hipStream_t stream)
+{
+    Shape s{};
+    hipError_t err = validate(d, n, stride, nb, s);
+    if (err != hipSuccess) return err;
+    Launch L;  // launch shape without launching: the range length of the summation order
+    err = kronmult_launch<T>(d, n, mats, stride, in, out, nullptr, nb, tuning, nullptr, &L, true);
+    if (err != hipSuccess) return err;
+    if (nb == 0) return hipSuccess;
+    if (mats == nullptr || in == nullptr || out == nullptr) return hipErrorInvalidValue;
+    int const epw = (L.workspace && use_passes<T>(s)) ? 1 : L.epw;  // multi-pass: every entry is a piece
+    long const ranges = (static_cast<long>(nb) + epw - 1) / epw;
+    long const slots = epw == 1 ? nb : 2 * ranges;
+    size_t const ptr_bytes = (static_cast<size_t>(nb) * sizeof(T *) + 255) / 256 * 256;
+    size_t const slot_bytes = static_cast<size_t>(s.size) * sizeof(T);
+    if (ptr_bytes > KRONMULT_NOALLOC_SCRATCH_BYTES ||
+        static_cast<size_t>(slots) > (KRONMULT_NOALLOC_SCRATCH_BYTES - ptr_bytes) / slot_bytes)
+        return hipErrorOutOfMemory;
+    void *base = nullptr;
+    err = scratch_address(&base);
+    if (err != hipSuccess) return err;
+    T **wsp = static_cast<T **>(base);
+    T *slot0 = reinterpret_cast<T *>(static_cast<unsigned char *>(base) + ptr_bytes);
+    hipLaunchKernelGGL(scratch_workspace_kernel<T>, dim3(static_cast<unsigned>((nb + 255) / 256)), dim3(256), 0,
+                       stream, nb, epw, ranges, static_cast<long>(s.size), wsp, slot0);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    return kronmult_launch<T>(d, n, mats, stride, in, out, wsp, nb, tuning, stream, nullptr, true);
+}
+
 } // namespace
 
 extern "C"
@@ -1252,6 +1327,15 @@ int kronmult_batched_ordered_f32_async(int matrix_count, int matrix_size, float 
     return static_cast<int>(kronmult_launch<float>(matrix_count, matrix_size, matrix_list_batched, matrix_stride,
                                                    input_batched, output_batched, workspace_batched, nb_batch,
                                                    tuning, static_cast<hipStream_t>(stream), nullptr, true));
+}
+
+int kronmult_batched_ordered_f64_noalloc(int matrix_count, int matrix_size, double const *const *matrix_list_batched,
+                                         int matrix_stride, double **input_batched, double **output_batched,
+                                         int nb_batch, kronmult_tuning_t const *tuning, void *stream)
+{
+    return static_cast<int>(kronmult_ordered_noalloc<double>(matrix_count, matrix_size, matrix_list_batched,
+                                                             matrix_stride, input_batched, output_batched, nb_batch,
+                                                             tuning, static_cast<hipStream_t>(stream)));
 }
 
 int kronmult_batched_terms_f64(int matrix_count, int matrix_size, int term_count,
