@@ -18,6 +18,8 @@
 //              Error per mode ~ 2^-(2b+24) of sum |A||x|: ~1e-13 relative for n <= 8.
 // The plain FP32/BF16 variants exist to report what the splitting buys; FIXED3 is the accurate one.
 //
+// This generic kernel serves every shape except FIXED3 with n = 4 on the register plans (d = 4, 5, 6), which
+// runs on kron_split_reg.hpp.
 // One workgroup (4 wavefronts) per entry; the entry's vector lives in LDS as FP64 (n^d * 8 <= 64 KiB). A
 // mode step takes tiles of 16 fibers: D (16 fibers x 16 outputs) = X^T (16 x 4) . A^T (4 x 16) per K-step of
 // 4, rows r >= n and k >= n zero-padded. Output accumulation into the caller's outputs: FP64 atomics.

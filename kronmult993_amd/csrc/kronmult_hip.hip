@@ -34,6 +34,7 @@
 #include "kron_ordered.hpp"
 #include "kron_pass.hpp"
 #include "kron_split.hpp"
+#include "kron_split_reg.hpp"
 #include "kron_reg.hpp"
 #include "kron_reg_adjoint.hpp"
 #include "kronmult_hip.h"
@@ -1058,6 +1059,25 @@ hipError_t split_dispatch_n(Shape const &s, double const *const *mats, double co
     }
 }
 
+// FIXED3 for the n = 4 register plans: the register-resident split sweep (kron_split_reg.hpp), runs of
+// entries per workgroup, one workgroup per resident slot.
+template<int D>
+hipError_t split_reg_launch(Shape const &s, double const *const *mats, double const *const *in,
+                            double *const *out, hipStream_t stream)
+{
+    using PL = kronplan::Plan<D, 4, reg_rho(D, 4)>;
+    auto kernel = kronsplitreg::kron_split_reg_kernel<PL>;
+    int const per_cu = resident_per_cu(kernel, PL::P, 0);
+    long const slots = static_cast<long>(device_cu_count()) * per_cu;
+    int epw = static_cast<int>((s.nb + slots - 1) / slots);
+    if (epw < 1) epw = 1;
+    long const runs = (s.nb + epw - 1) / epw;
+    long const grid = (runs + 7) / 8 * 8;
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(grid)), dim3(PL::P), 0, stream, s.nb, epw, mats,
+                       s.stride, in, out);
+    return hipGetLastError();
+}
+
 hipError_t kronmult_split_launch(int d, int n, double const *const *mats, int stride, double *const *in,
                                  double *const *out, int nb, int mode, hipStream_t stream)
 {
@@ -1069,6 +1089,12 @@ hipError_t kronmult_split_launch(int d, int n, double const *const *mats, int st
     if (nb == 0) return hipSuccess;
     if (mats == nullptr || in == nullptr || out == nullptr) return hipErrorInvalidValue;
     double const *const *cin = const_cast<double const *const *>(in);
+    if (mode == KRONMULT_SPLIT_FIXED3 && n == 4)
+    {
+        if (d == 6) return split_reg_launch<6>(s, mats, cin, out, stream);
+        if (d == 5) return split_reg_launch<5>(s, mats, cin, out, stream);
+        if (d == 4) return split_reg_launch<4>(s, mats, cin, out, stream);
+    }
     switch (mode)
     {
     case KRONMULT_SPLIT_FP32: return split_dispatch_n<kronsplit::FP32>(s, mats, cin, out, stream);

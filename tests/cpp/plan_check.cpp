@@ -88,6 +88,21 @@ void check_plan(char const *name, bool conflict_free)
         for (int e = 0; e < S; ++e) ok = ok && seen[e] == 1;
         CHECK(ok, "%s phase %d layout is not a bijection", name, p);
     }
+    // 1b. thread part | register part: the element index is their OR (disjoint bits) and each swizzle maps it
+    // to phys(thread part) ^ phys(register part) -- the split sweep's exchange addressing relies on it
+    for (int p = 0; p < PL::NPHASE; ++p)
+    {
+        bool ok = true;
+        for (int t = 0; t < P; ++t)
+            for (int j = 0; j < R; ++j)
+            {
+                int const a = PL::elem(p, t, 0), b = PL::elem(p, 0, j), e = PL::elem(p, t, j);
+                ok = ok && (a & b) == 0 && (a | b) == e;
+                ok = ok && SW_IN::phys(e) == (SW_IN::phys(a) ^ SW_IN::phys(b));
+                ok = ok && SW_EX::phys(e) == (SW_EX::phys(a) ^ SW_EX::phys(b));
+            }
+        CHECK(ok, "%s phase %d: addresses are not phys(thread part) ^ phys(register part)", name, p);
+    }
     for (int stage = 0; stage < 2; ++stage)
     {
         auto phys = [&](int e) { return stage == 0 ? SW_IN::phys(e) : SW_EX::phys(e); };

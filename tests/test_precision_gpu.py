@@ -63,7 +63,7 @@ def _fp32(b, gpu):
     return db.output()
 
 
-@pytest.mark.parametrize("d,n", [(1, 3), (2, 8), (3, 5), (3, 8), (4, 4), (5, 3), (6, 2), (6, 4), (4, 7)])
+@pytest.mark.parametrize("d,n", [(1, 3), (2, 8), (3, 5), (3, 8), (4, 4), (5, 3), (5, 4), (6, 2), (6, 4), (4, 7)])
 def test_split_fixed3_random(gpu, d, n):
     b = W.random_case(seed=50 + d * n, d=d, n=n, nb=23)
     mx, rms = errors(_split(b, gpu, "fixed3"), oracle.kronmult_hp(b))
@@ -77,6 +77,40 @@ def test_split_modes(gpu, config, mode):
     mx, rms = errors(_split(b, gpu, mode), ref)
     print(f"[split] {config} {mode}: max {mx:.3e} rms {rms:.3e}")
     assert mx <= SPLIT_BOUND[mode], (mx, rms)
+
+
+@pytest.mark.parametrize("d", [4, 5, 6])
+def test_split_register_sweep_n4(gpu, d):
+    """The register-resident split sweep (the n = 4 plans): runs of entries into scattered shared outputs,
+    padded strides, factors spanning 10^-3 .. 10^3, an all-zero factor row and an all-zero input vector,
+    against the extended-precision oracle."""
+    from helpers import factors
+    b = W.random_case(seed=70 + d, d=d, n=4, nb=300, distinct=17)
+    pool = b.mat_pool.copy()
+    scale = 10.0 ** np.linspace(-3, 3, b.nb * d)
+    for k in range(b.nb):
+        for t in range(d):
+            off = int(b.mat_off[k * d + t])
+            for c in range(4):
+                pool[off + c * b.stride: off + c * b.stride + 4] = factors(b, k)[t][:, c] * scale[k * d + t]
+    off = int(b.mat_off[5 * d + 1])
+    for c in range(4):
+        pool[off + 2 + c * b.stride] = 0.0  # row 2 of factor 1 of entry 5
+    inp = b.in_pool.copy()
+    inp[int(b.in_off[7]): int(b.in_off[7]) + b.size] = 0.0
+    b = W.HostBatch(**{**b.__dict__, "mat_pool": pool, "in_pool": inp})
+    assert api.kernel_name(d, 4, 8, b.nb).startswith("reg_sweep")
+    mx, rms = errors(_split(b, gpu, "fixed3"), oracle.kronmult_hp(b))
+    assert mx <= 1e-12, (mx, rms)
+
+
+def test_split_register_sweep_misaligned_inputs(gpu):
+    """d=6 n=4 with input pointers alternating between 16-byte and element alignment along every run: the
+    LDS-DMA staging and its register fallback switch from one entry to the next."""
+    from helpers import shift_odd_inputs
+    b = shift_odd_inputs(W.c2(rows=3, cols=64))
+    mx, rms = errors(_split(b, gpu, "fixed3"), oracle.kronmult(b))
+    assert mx <= 1e-12, (mx, rms)
 
 
 def test_split_c4_dense_subset(gpu):
