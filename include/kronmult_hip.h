@@ -54,7 +54,8 @@ int kronmult_batched_f32_async(int matrix_count, int matrix_size, float const *c
  * single launch shape, kronmult_gpu/kronmult.cu:184-193). Zero fields mean "library default".
  *   family:                 KRONMULT_FAMILY_AUTO, _REGISTER (register-resident sweep, only for the
  *                           (d, n) plans it is built for), _FIBER (fiber sweep, one vector per
- *                           workgroup, any n <= 64), _MATRIX (every contraction on the FP64 matrix cores;
+ *                           workgroup, any n with n * sizeof(T) <= 32 KiB; n > 64 on the wide-fiber
+ *                           kernel), _MATRIX (every contraction on the FP64 matrix cores;
  *                           d = 6, n = 4, double only) or _PACKED (fiber sweep with several small vectors per workgroup: n <= 8,
  *                           n^(d-1) <= 64; the default for such shapes from 20 entries per CU on);
  *   workgroups_per_cu:      workgroups to launch per compute unit (register, matrix, packed families);

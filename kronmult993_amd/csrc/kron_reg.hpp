@@ -146,7 +146,11 @@ struct Coef
 {
     static constexpr int kCount = PL::kD * PL::kN * PL::kN;                  // staged values per entry
     static constexpr int kPerThread = (kCount + PL::P - 1) / PL::P;
-    // staging layout: factor m, row r, column k at m*N*N + r*N + k
+    // staging layout: factor m, row r, column k at m*N*N + r*N + k.
+    // When N*N is a multiple of 64 the factor index m of a thread's slot is the same across its wavefront, so
+    // the factor pointer comes through a scalar load. A vector pointer load would make the dependent value
+    // load wait for every vector load issued before it -- the next entry's input prefetch included, which
+    // then stalls the entry instead of overlapping it.
     static __device__ __forceinline__ void load(T const *const *A_list, int stride, T (&pre)[kPerThread], int t)
     {
 #pragma unroll
@@ -155,7 +159,8 @@ struct Coef
             int const idx = t + i * PL::P;
             if (kCount % PL::P == 0 || idx < kCount)
             {
-                int const m = idx / (PL::kN * PL::kN);
+                int m = idx / (PL::kN * PL::kN);
+                if constexpr ((PL::kN * PL::kN) % 64 == 0 && PL::P % 64 == 0) m = __builtin_amdgcn_readfirstlane(m);
                 int const r = (idx / PL::kN) % PL::kN;
                 int const k = idx % PL::kN;
                 pre[i] = as_global(A_list[m])[r + k * stride];
@@ -311,6 +316,63 @@ struct Lds
 #pragma unroll
             for (int j = 0; j < PL::R; ++j) v[j] = lds[SW::phys(PL::elem(PH, t, j))];
         }
+    }
+
+    // Exchange with addresses formed at each access (the split sweep, kron_split_reg.hpp). The swizzles are
+    // GF(2)-linear in the element index and a phase layout's element index is the OR of a thread part and a
+    // register part (disjoint bit fields), so the byte address of register j is phys(thread part) ^
+    // phys(register part j): one base register per access pattern and a compile-time constant per register
+    // (verified for every plan by tests/cpp/plan_check.cpp). The base is made opaque inside the entry loop,
+    // so the compiler forms the R addresses at the accesses instead of keeping them live across the loop (R
+    // registers per pattern). Measured in the FP64 sweeps it gains nothing: their addresses fit.
+    template<int PH>
+    struct XorAddr
+    {
+        static constexpr unsigned reg(int j)
+        {
+            return static_cast<unsigned>(SW_EX::phys(PL::elem(PH, 0, j)) * static_cast<int>(sizeof(T)));
+        }
+        static __device__ __forceinline__ unsigned base(int t)
+        {
+            unsigned b = static_cast<unsigned>(SW_EX::phys(PL::elem(PH, t, 0)) * static_cast<int>(sizeof(T)));
+            asm volatile("" : "+v"(b));
+            return b;
+        }
+    };
+
+    template<int PH>
+    static __device__ __forceinline__ void write_xor(T *lds, T const (&v)[PL::R], int t)
+    {
+        using A = XorAddr<PH>;
+        char *const bytes = reinterpret_cast<char *>(lds);
+        unsigned const b = A::base(t);
+        if constexpr (PH == 0 && kVec)
+        {
+#pragma unroll
+            for (int q = 0; q < PL::R / G; ++q)
+            {
+                V g;
+#pragma unroll
+                for (int i = 0; i < G; ++i) g[i] = v[q * G + i];
+                *reinterpret_cast<V *>(bytes + (b ^ A::reg(q * G))) = g;
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int j = 0; j < PL::R; ++j) *reinterpret_cast<T *>(bytes + (b ^ A::reg(j))) = v[j];
+        }
+    }
+
+    template<int PH>
+    static __device__ __forceinline__ void read_xor(T const *lds, T (&v)[PL::R], int t)
+    {
+        static_assert(PH >= 1, "phase 0 reads the staged input");
+        using A = XorAddr<PH>;
+        char const *const bytes = reinterpret_cast<char const *>(lds);
+        unsigned const b = A::base(t);
+#pragma unroll
+        for (int j = 0; j < PL::R; ++j) v[j] = *reinterpret_cast<T const *>(bytes + (b ^ A::reg(j)));
     }
 
     template<int PH>

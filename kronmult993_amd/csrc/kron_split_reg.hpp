@@ -228,62 +228,6 @@ __device__ __forceinline__ void contract_mode(double (&v)[R], Factor const &F)
     }
 }
 
-// LDS exchange with addresses formed at each access. The swizzles are GF(2)-linear in the element index and
-// a phase layout's element index is the OR of a thread part and a register part (disjoint bit fields), so
-// the byte address of register j is phys(thread part) ^ phys(register part j): one base register per access
-// pattern and a compile-time constant per register (verified for every plan by tests/cpp/plan_check.cpp).
-// The base is made opaque inside the entry loop, so the compiler forms the R addresses at the accesses
-// instead of keeping them live across the loop (R registers per pattern).
-template<typename PL, typename SW, int PH>
-struct XorAddr
-{
-    static constexpr unsigned reg(int j) { return static_cast<unsigned>(SW::phys(PL::elem(PH, 0, j)) * 8); }
-    static __device__ __forceinline__ unsigned base(int t)
-    {
-        unsigned b = static_cast<unsigned>(SW::phys(PL::elem(PH, t, 0)) * 8);
-        asm volatile("" : "+v"(b));
-        return b;
-    }
-};
-
-template<typename PL, int PH>
-__device__ __forceinline__ void exchange_write(double *lds, double const (&v)[PL::R], int t)
-{
-    using SW = typename Lds<double, PL>::SW_EX;
-    using A = XorAddr<PL, SW, PH>;
-    char *const bytes = reinterpret_cast<char *>(lds);
-    unsigned const b = A::base(t);
-    if constexpr (PH == 0 && Lds<double, PL>::kVec)
-    {
-        using V = typename Lds<double, PL>::V;
-#pragma unroll
-        for (int q = 0; q < PL::R / 2; ++q)
-        {
-            V g;
-            g[0] = v[2 * q];
-            g[1] = v[2 * q + 1];
-            *reinterpret_cast<V *>(bytes + (b ^ A::reg(2 * q))) = g;
-        }
-    }
-    else
-    {
-#pragma unroll
-        for (int j = 0; j < PL::R; ++j) *reinterpret_cast<double *>(bytes + (b ^ A::reg(j))) = v[j];
-    }
-}
-
-template<typename PL, int PH>
-__device__ __forceinline__ void exchange_read(double const *lds, double (&v)[PL::R], int t)
-{
-    static_assert(PH >= 1, "phase 0 reads the staged input");
-    using SW = typename Lds<double, PL>::SW_EX;
-    using A = XorAddr<PL, SW, PH>;
-    char const *const bytes = reinterpret_cast<char const *>(lds);
-    unsigned const b = A::base(t);
-#pragma unroll
-    for (int j = 0; j < PL::R; ++j) v[j] = *reinterpret_cast<double const *>(bytes + (b ^ A::reg(j)));
-}
-
 // contract modes M, M-1, ..., LO of phase PH (fastest first), each with its factor read from the table just
 // before use (the table is double-buffered per entry, so the reads need no barrier of their own)
 template<typename PL, int PH, int M>
@@ -319,9 +263,9 @@ __device__ __forceinline__ void run_phases(double *lds, double (&v)[PL::R], unsi
             if (has_next) prep.store(next_tab, t);
         }
         lds_barrier();  // every thread has read phase PH
-        exchange_write<PL, PH>(lds, v, t);
+        Lds<double, PL>::template write_xor<PH>(lds, v, t);
         lds_barrier();
-        exchange_read<PL, PH + 1>(lds, v, t);
+        Lds<double, PL>::template read_xor<PH + 1>(lds, v, t);
         run_phases<PL, PH + 1>(lds, v, tab, next_tab, prep, has_next, next_mats, stride, t, after_last_read);
     }
 }

@@ -42,7 +42,7 @@
 namespace
 {
 
-constexpr int kMaxFiber = 64;           // largest n handled (runtime-n kernel)
+constexpr int kMaxFiber = 64;           // largest n of the per-thread fiber kernels (wide-fiber kernel beyond)
 constexpr int kLdsBudget = 64 * 1024;   // bytes of LDS one workgroup may take for the n^d vector
 constexpr int kFiberThreads = 256;      // workgroup size of the fiber sweep
 
@@ -53,7 +53,7 @@ constexpr int kFiberThreads = 256;      // workgroup size of the fiber sweep
 // fiber {base + k*st, k < n} to {base + r*st, r < n} with y_r = sum_k A_t[r, k] * x_k. There are n^(d-1)
 // fibers per step; fiber f has base = (f / st) * st * n + f % st.
 //
-// N > 0: compile-time fiber length, coefficients held in registers (uniform across the workgroup, so the
+// N > 0 (n <= 10): compile-time fiber length, coefficients held in registers (uniform across the workgroup, so the
 // compiler loads them with scalar loads). N == 0: runtime n <= kMaxFiber, coefficients staged in LDS.
 // InLds: the vector lives in LDS; otherwise in ws[b] (the first step reads the input directly).
 // ORDERED: each entry is a piece (kron_ordered.hpp): added to its output when it is the output's only
@@ -145,6 +145,80 @@ __global__ void __launch_bounds__(kFiberThreads)
         else if (InLds)
         {
             T *w = ws[b];
+            for (int i = threadIdx.x; i < size; i += blockDim.x) w[i] = x[i];
+        }
+    }
+    else
+    {
+        for (int i = threadIdx.x; i < size; i += blockDim.x) atomicAdd(y + i, x[i]);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Wide-fiber sweep (n > kMaxFiber, any d): one workgroup per entry, no per-thread fiber copy. Each mode is
+// contracted in chunks of G fibers: the chunk's n*G inputs are staged in LDS (thread idx -> fiber idx % G,
+// element idx / G, so consecutive threads read consecutive fibers: contiguous in memory when G <= stride),
+// then every (fiber, row) output is one fused chain over k ascending from zero -- the reference's step
+// (kronmult_gpu/kronmult.cu:66-70) -- written in place into the entry's vector, which lives in LDS when it
+// fits next to the chunk, else in the caller's workspace (the first mode reads the input). Coefficients
+// come from the factor itself (global memory, read through the caches: n*n values shared by the workgroup).
+// ------------------------------------------------------------------------------------------------------
+constexpr int kWideThreads = 256;
+constexpr int kWideChunkBytes = 32 * 1024;  // staged fibers per chunk (n <= 4096 in FP64, 8192 in FP32)
+
+template<typename T, bool InLds, bool ORDERED>
+__global__ void __launch_bounds__(kWideThreads)
+    kron_wide_kernel(int d, int n, int size, int nb, int chunk, T const *const *__restrict__ mats, int stride,
+                     T const *const *__restrict__ in, T *const *__restrict__ out, T *const *__restrict__ ws)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int const b = blockIdx.x;
+    T *const buf = reinterpret_cast<T *>(smem_raw);             // chunk * n staged fiber values
+    T *const x = InLds ? buf + static_cast<size_t>(chunk) * n : ws[b];
+    T const *const src = in[b];
+    T const *const *A_list = mats + static_cast<size_t>(b) * d;
+    int const nfib = size / n;
+    int st = 1;
+    for (int t = d - 1; t >= 0; --t)
+    {
+        T const *const A = A_list[t];
+        T const *const rd = (t == d - 1) ? src : x;
+        for (int f0 = 0; f0 < nfib; f0 += chunk)
+        {
+            int const g = (nfib - f0 < chunk) ? nfib - f0 : chunk;
+            __syncthreads();  // previous chunk (or mode, or the input copy) done with buf / x
+            for (int i = threadIdx.x; i < g * n; i += blockDim.x)
+            {
+                int const fl = i % g, k = i / g;
+                int const f = f0 + fl;
+                int const hi = f / st;
+                buf[fl * n + k] = rd[hi * st * n + (f - hi * st) + k * st];
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < g * n; i += blockDim.x)
+            {
+                int const fl = i % g, r = i / g;
+                int const f = f0 + fl;
+                int const hi = f / st;
+                T const *const xf = buf + fl * n;
+                T acc = T(0);
+                for (int k = 0; k < n; ++k) acc = fma(xf[k], A[r + static_cast<size_t>(k) * stride], acc);
+                x[hi * st * n + (f - hi * st) + r * st] = acc;
+            }
+        }
+        st *= n;
+    }
+    __syncthreads();
+    T *const y = out[b];
+    if constexpr (ORDERED)
+    {
+        if (kronordered::whole_segment(out, nb, b, b + 1, y))
+        {
+            for (int i = threadIdx.x; i < size; i += blockDim.x) y[i] = y[i] + x[i];
+        }
+        else if (InLds)
+        {
+            T *const w = ws[b];
             for (int i = threadIdx.x; i < size; i += blockDim.x) w[i] = x[i];
         }
     }
@@ -258,7 +332,10 @@ hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T 
     L.block = PL::P;
     L.lds = sizeof(T) * (PL::SIZE + (kronreg::fwd_coef_in_lds<T, PL>() ? kronreg::Coef<T, PL>::kCount : 0));
     L.resident = resident_per_cu(kernel, PL::P, 0);
-    int const per_cu = tu.workgroups_per_cu > 0 ? tu.workgroups_per_cu : default_workgroups_per_cu(PL::P);
+    // by default no more workgroups than fit at once: a second round of workgroups starts only as the first
+    // drains (measured on the d=4 n=8 FP32 batch: 3 resident, 118 us at 3 per CU against 149 us at 4)
+    int const per_cu = tu.workgroups_per_cu > 0 ? tu.workgroups_per_cu
+                                                : std::min(default_workgroups_per_cu(PL::P), L.resident);
     long const slots = static_cast<long>(device_cu_count()) * per_cu;
     int epw = tu.entries_per_workgroup > 0 ? tu.entries_per_workgroup : static_cast<int>((s.nb + slots - 1) / slots);
     if (epw < 1) epw = 1;
@@ -354,11 +431,15 @@ hipError_t fiber_dispatch_n(Shape const &s, Launch &L, T const *const *mats, T c
     case 6: return fiber_launch<T, 6, InLds>(s, L, mats, in, out, ws, stream, run);
     case 7: return fiber_launch<T, 7, InLds>(s, L, mats, in, out, ws, stream, run);
     case 8: return fiber_launch<T, 8, InLds>(s, L, mats, in, out, ws, stream, run);
+    case 9: return fiber_launch<T, 9, InLds>(s, L, mats, in, out, ws, stream, run);
+    case 10: return fiber_launch<T, 10, InLds>(s, L, mats, in, out, ws, stream, run);
     default: return fiber_launch<T, 0, InLds>(s, L, mats, in, out, ws, stream, run);
     }
 }
 
 // ---- multi-pass family (vectors beyond the on-chip size, n <= 8) ---------------------------------------
+// n <= 8: for n = 9, 10 the pass kernels' coefficients overflow the scalar registers, and the fiber sweep
+// with a compile-time fiber length is faster (measured, 4608 entries: d=5 n=9 5.9 ms against 42 ms)
 constexpr int kPassMaxN = 8;
 
 // Largest power of n not above `limit` (at least 1).
@@ -485,11 +566,62 @@ bool use_passes(Shape const &s)
     return !fiber_in_lds<T>(s) && s.n >= 2 && s.n <= kPassMaxN;
 }
 
+// ---- wide-fiber family (n > kMaxFiber) -----------------------------------------------------------------
+template<typename T>
+bool wide_ok(Shape const &s)
+{
+    return static_cast<long>(s.n) * static_cast<long>(sizeof(T)) <= kWideChunkBytes;
+}
+
+// chunk of fibers staged per step, and whether the vector fits in LDS beside it
+template<typename T>
+void wide_shape(Shape const &s, int &chunk, bool &in_lds)
+{
+    long const nfib = s.size / s.n;
+    long const fiber_bytes = static_cast<long>(s.n) * static_cast<long>(sizeof(T));
+    long g = kWideChunkBytes / fiber_bytes;
+    if (g > nfib) g = nfib;
+    chunk = static_cast<int>(g < 1 ? 1 : g);
+    in_lds = s.size * static_cast<long>(sizeof(T)) + chunk * fiber_bytes <= kLdsBudget;
+}
+
+template<typename T, bool InLds>
+hipError_t wide_launch(Shape const &s, Launch &L, int chunk, T const *const *mats, T const *const *in,
+                       T *const *out, T *const *ws, hipStream_t stream, bool run)
+{
+    size_t const lds = static_cast<size_t>(chunk) * s.n * sizeof(T) + (InLds ? static_cast<size_t>(s.size) * sizeof(T) : 0);
+    auto kernel = s.ordered ? kron_wide_kernel<T, InLds, true> : kron_wide_kernel<T, InLds, false>;
+    L.family = KRONMULT_FAMILY_FIBER;
+    L.grid = s.nb;
+    L.block = kWideThreads;
+    L.epw = 1;
+    L.lds = lds;
+    L.workspace = !InLds;
+    L.resident = resident_per_cu(kernel, kWideThreads, lds);
+    if (!run) return hipSuccess;
+    hipLaunchKernelGGL(kernel, dim3(s.nb), dim3(kWideThreads), lds, stream, s.d, s.n, static_cast<int>(s.size), s.nb,
+                       chunk, mats, s.stride, in, out, ws);
+    return hipGetLastError();
+}
+
+template<typename T>
+hipError_t wide_dispatch(Shape const &s, Launch &L, T const *const *mats, T const *const *in, T *const *out,
+                         T *const *ws, hipStream_t stream, bool run)
+{
+    if (!wide_ok<T>(s)) return hipErrorInvalidValue;
+    int chunk = 1;
+    bool in_lds = false;
+    wide_shape<T>(s, chunk, in_lds);
+    if (in_lds) return wide_launch<T, true>(s, L, chunk, mats, in, out, ws, stream, run);
+    if (run && ws == nullptr) return hipErrorInvalidValue;
+    return wide_launch<T, false>(s, L, chunk, mats, in, out, ws, stream, run);
+}
+
 template<typename T>
 hipError_t fiber_dispatch(Shape const &s, Launch &L, T const *const *mats, T const *const *in, T *const *out,
                           T *const *ws, hipStream_t stream, bool run)
 {
-    if (s.n > kMaxFiber) return hipErrorInvalidValue;
+    if (s.n > kMaxFiber) return wide_dispatch<T>(s, L, mats, in, out, ws, stream, run);
     if (fiber_in_lds<T>(s)) return fiber_dispatch_n<T, true>(s, L, mats, in, out, ws, stream, run);
     if (run && ws == nullptr) return hipErrorInvalidValue;
     if (use_passes<T>(s)) return pass_dispatch<T>(s, L, mats, in, out, ws, stream, run);
@@ -594,13 +726,14 @@ int choose_family(Shape const &s, kronmult_tuning_t const &tu)
     bool const reg_ok = reg_rho(s.d, s.n) > 0;
     bool const mfma_ok = sizeof(T) == 8 && s.d == 6 && s.n == 4;
     if (tu.family == KRONMULT_FAMILY_REGISTER) return reg_ok ? KRONMULT_FAMILY_REGISTER : -1;
-    if (tu.family == KRONMULT_FAMILY_FIBER) return s.n <= kMaxFiber ? KRONMULT_FAMILY_FIBER : -1;
+    bool const fiber_ok = s.n <= kMaxFiber || wide_ok<T>(s);
+    if (tu.family == KRONMULT_FAMILY_FIBER) return fiber_ok ? KRONMULT_FAMILY_FIBER : -1;
     if (tu.family == KRONMULT_FAMILY_MATRIX) return mfma_ok ? KRONMULT_FAMILY_MATRIX : -1;
     if (tu.family == KRONMULT_FAMILY_PACKED) return packed_ok<T>(s) ? KRONMULT_FAMILY_PACKED : -1;
     if (tu.family != KRONMULT_FAMILY_AUTO) return -1;
     if (reg_ok) return KRONMULT_FAMILY_REGISTER;
     if (packed_ok<T>(s) && s.nb >= kPackedEntriesPerCu * device_cu_count()) return KRONMULT_FAMILY_PACKED;
-    return s.n <= kMaxFiber ? KRONMULT_FAMILY_FIBER : -1;
+    return fiber_ok ? KRONMULT_FAMILY_FIBER : -1;
 }
 
 // The reduction of an ordered call (kron_ordered.hpp): ranges of L.epw entries (the pieces' length), every
@@ -730,6 +863,14 @@ char const *kernel_name(int d, int n, int nb)
     case KRONMULT_FAMILY_PACKED: return f64 ? "packed_f64" : "packed_f32";
     case KRONMULT_FAMILY_REGISTER: return f64 ? "reg_sweep_f64" : "reg_sweep_f32";
     case KRONMULT_FAMILY_FIBER:
+        if (s.n > kMaxFiber)
+        {
+            int chunk = 1;
+            bool in_lds = false;
+            wide_shape<T>(s, chunk, in_lds);
+            if (in_lds) return f64 ? "wide_lds_f64" : "wide_lds_f32";
+            return f64 ? "wide_global_f64" : "wide_global_f32";
+        }
         if (fiber_in_lds<T>(s)) return f64 ? "fiber_lds_f64" : "fiber_lds_f32";
         if (use_passes<T>(s)) return f64 ? "multipass_f64" : "multipass_f32";
         return f64 ? "fiber_global_f64" : "fiber_global_f32";

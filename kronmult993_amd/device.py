@@ -40,7 +40,7 @@ class DeviceBatch:
         self.mat_ptrs = self._ptrs(self.mat, hb.mat_off)
         self.in_ptrs = self._ptrs(self.inp, hb.in_off)
         self.out_ptrs = self._ptrs(self.out, hb.out_off)
-        uses_ws = api.kernel_name(self.d, self.n, self.itemsize, self.nb).startswith(("fiber_global", "multipass"))
+        uses_ws = api.kernel_name(self.d, self.n, self.itemsize, self.nb).startswith(("fiber_global", "multipass", "wide_global"))
         need_ws = workspace == "per_entry" or (workspace == "auto" and uses_ws)
         if need_ws:
             self.ws = torch.zeros(self.nb * self.size, dtype=self.out.dtype, device=self.device)

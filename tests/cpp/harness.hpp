@@ -76,7 +76,10 @@ struct DeviceBatch
     std::vector<T *> hp_in, hp_out, hp_ws;
     std::vector<int> out_index;
 
-    DeviceBatch(int d_, int n_, int stride_, int nb_, int distinct_, uint64_t seed, bool fill, bool workspace)
+    // fill: seeded N(0,1) data; tiled: the first 2^16 values of each role repeated (timing runs of large
+    // batches, where generating every value would take tens of seconds)
+    DeviceBatch(int d_, int n_, int stride_, int nb_, int distinct_, uint64_t seed, bool fill, bool workspace,
+                bool tiled = false)
         : d(d_), n(n_), stride(stride_), nb(nb_), distinct(distinct_ < nb_ ? distinct_ : nb_)
     {
         size = 1;
@@ -88,9 +91,14 @@ struct DeviceBatch
         if (fill)
         {
             Stream sm(seed, 20), si(seed, 21), so(seed, 22);
-            for (auto &x : h_mats) x = static_cast<T>(sm.normal());
-            for (auto &x : h_in) x = static_cast<T>(si.normal());
-            for (auto &x : h_out) x = static_cast<T>(so.normal());
+            auto gen = [&](std::vector<T> &v, Stream &st) {
+                size_t const m = tiled && v.size() > 65536 ? 65536 : v.size();
+                for (size_t i = 0; i < m; ++i) v[i] = static_cast<T>(st.normal());
+                for (size_t i = m; i < v.size(); ++i) v[i] = v[i % m];
+            };
+            gen(h_mats, sm);
+            gen(h_in, si);
+            gen(h_out, so);
         }
         check(hipMalloc(&mats, sizeof(T) * mat_elems), "hipMalloc mats");
         check(hipMalloc(&in, sizeof(T) * h_in.size()), "hipMalloc in");

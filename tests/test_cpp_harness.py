@@ -43,9 +43,17 @@ def test_cpp_kronmult_bench_gpu(gpu):
     assert "GFLOP/s" in p.stdout
 
 
+@pytest.mark.gpu
+def test_cpp_kronmult_fullbench_gpu(gpu):
+    """The full-grid benchmark on its small corner (degrees 2-3, all dimensions, level 2)."""
+    p = _run(["kronmult_fullbench_gpu", "2", "2", "2", "3"], 120)
+    assert p.returncode == 0, p.stderr
+    assert p.stdout.count("GFLOP/s") == 12 and "fullbench done" in p.stdout
+
+
 def test_cpp_harness_built():
     """The executables link against the product library (no GPU needed to check the link)."""
-    for exe in ("kronmult_test_gpu", "kronmult_bench_gpu"):
+    for exe in ("kronmult_test_gpu", "kronmult_bench_gpu", "kronmult_fullbench_gpu"):
         path = os.path.join(BIN, exe)
         if not os.path.exists(path):
             pytest.skip("tests/cpp not built")

@@ -132,7 +132,10 @@ def test_kernel_selection():
     assert api.kernel_name(4, 5, 4, 6000).startswith("fiber_lds")  # 125 fibers: fills a workgroup alone
     assert api.kernel_name(6, 8, 8, 3903) == "multipass_f64"
     assert api.kernel_name(3, 21, 8, 6) == "fiber_global_f64"
-    assert api.kernel_name(2, 100, 8, 1) == "unsupported"
+    assert api.kernel_name(2, 100, 8, 1) == "wide_global_f64"  # n > 64: wide fibers, vector in the workspace
+    assert api.kernel_name(1, 100, 8, 1) == "wide_lds_f64"
+    assert api.kernel_name(1, 4097, 8, 1) == "unsupported"    # a fiber above the 32 KiB staging chunk
+    assert api.kernel_name(1, 8192, 4, 1) == "wide_lds_f32" or api.kernel_name(1, 8192, 4, 1) == "wide_global_f32"
     assert "gfx950" in api.version()
 
 

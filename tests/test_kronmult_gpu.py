@@ -122,6 +122,28 @@ def test_random_shapes(gpu, d, n, nb, seed):
         _check_bitexact(u, _run(u, gpu))
 
 
+@pytest.mark.parametrize("d,n,nb,seed", [(1, 65, 9, 21), (1, 300, 5, 22), (2, 70, 12, 23), (2, 129, 6, 24),
+                                         (3, 65, 3, 25), (2, 66, 40, 26)])
+def test_large_n_wide_fibers(gpu, d, n, nb, seed):
+    """n > 64 (the reference takes any n, kronmult_gpu/kronmult.cu:54-78): the wide-fiber kernel, vector in
+    LDS (d=1) or in the caller's workspace, padded strides, shared factors, aliased outputs; bit-exact per
+    entry, in FP64 and FP32."""
+    for dtype in DTYPES:
+        b = W.random_case(seed, d, n, nb, shared_factors=max(1, nb // 3)).astype(dtype)
+        assert api.kernel_name(d, n, b.dtype.itemsize, b.nb).startswith("wide_")
+        _check_aliased(b, _run(b, gpu))
+        u = _unaliased(b)
+        _check_bitexact(u, _run(u, gpu))
+
+
+def test_large_n_limit(gpu):
+    """A fiber must fit the 32 KiB staging chunk: n = 4097 in FP64 is refused, n = 1500 (d = 1) runs."""
+    with pytest.raises(api.KronmultError):
+        api.launch_info(1, 4097, 8, 1)
+    u = _unaliased(W.random_case(27, 1, 1500, 2))
+    _check_bitexact(u, _run(u, gpu))
+
+
 @pytest.mark.parametrize("d,n", [(6, 4), (5, 4), (4, 4), (4, 8), (3, 8)])
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_register_plans_irregular(gpu, d, n, dtype):

@@ -69,9 +69,10 @@ def test_ordered_irregular_batches(gpu, d, n, nb, dtype):
     _check(b, gpu)
 
 
-@pytest.mark.parametrize("d,n,nb", [(3, 21, 6), (6, 8, 4), (6, 7, 3)])
+@pytest.mark.parametrize("d,n,nb", [(3, 21, 6), (6, 8, 4), (6, 7, 3), (2, 70, 5), (1, 100, 7)])
 def test_ordered_large_vectors(gpu, d, n, nb):
-    """Workspace fiber sweep (n > 8) and the multi-pass sweep (every entry reduced from its workspace)."""
+    """Workspace fiber sweep (n > 8), the multi-pass sweep (every entry reduced from its workspace) and the
+    wide-fiber sweep (n > 64, workspace and LDS)."""
     b = grouped_by_output(W.random_case(95 + d + n, d, n, nb, distinct=2))
     _check(b, gpu)
 
