@@ -12,10 +12,11 @@ throughput without host launch gaps. A separate eager pass records HIP events ar
 same stream; their mean is the kernel duration used for the roofline.
 
 Printed: one JSON line with GFLOP/s (value), the binding roofline of the kernel (`roofline`: compute --
-flops per launch over the kernel duration against the FP64 / FP32 peak -- when the flops at peak take
-longer than the bytes that move at 8 TB/s, else HBM -- contract bytes per launch over the kernel duration),
-both rooflines (`roofline_compute`, `roofline_hbm`), the HBM traffic per launch measured with rocprofv3
-counters (profiles/traffic.json), and the reference's CPU implementation timed on this host.
+flops per launch over the kernel duration against the peak of the pipe that does them -- when the flops at
+that peak take longer than the measured bytes at 8 TB/s, else HBM -- the HBM bytes per launch measured with
+rocprofv3 counters (profiles/traffic.json) over the kernel duration), both rooflines (`roofline_compute`,
+`roofline_hbm`), the per-entry contract bytes over the kernel duration (`contract_gbs`, not a fraction of
+anything), the split-precision leg (`split`) and the reference's CPU implementation timed on this host.
 
     python bench.py [--steps K] [--warmup W] [--config c2] [--dtype f64] [--op forward|adjoint|split|ordered|terms]
                     [--intermediates auto|recompute|store] [--cpu-baseline auto|off] [--calibrate-copy]

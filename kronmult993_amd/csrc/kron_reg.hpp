@@ -614,6 +614,19 @@ __global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(min_
         return;
     }
 
+    // entry e goes to another output than the run so far: flush the run
+    auto flush_if_new = [&](int e) {
+        T *dst = out[e];
+        if (dst != cur)
+        {
+            if constexpr (ORDERED)
+                flush_piece<T, PL>(cur, acc, t, out, ws, nb, ps, e);
+            else
+                flush<T, PL>(cur, acc, t);
+            cur = dst;
+            ps = e;
+        }
+    };
     T pre[PL::R];
     T pre_coef[kCoefLds ? Coef<T, PL>::kPerThread : 1];
     load_input<T, PL>(in[e0], pre, t);
@@ -621,6 +634,12 @@ __global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(min_
 
     for (int e = e0; e < e1; ++e)
     {
+        // With factors staged in LDS the previous run is flushed before the sweep, not after it: with the
+        // branch between the last phase and the accumulation, the compiler issued that phase's coefficient
+        // reads (64 values for n = 8 FP64) above the branch and its FMAs below, holding all of them live
+        // (220 VGPRs; 120 flushing first; d=4 n=8: 236 -> 210 us per launch). The other plans flush after
+        // the sweep (FP32 d=6 n=4: 34.2 against 36.2 us flushing first).
+        if constexpr (kCoefLds) flush_if_new(e);
         lds_barrier();  // LDS free: the previous entry's last phase has been read by everyone
         Lds<T, PL>::store_input(lds, pre, t);
         if constexpr (kCoefLds) Coef<T, PL>::store(coef, pre_coef, t);
@@ -633,17 +652,7 @@ __global__ void __launch_bounds__(PL::P) __attribute__((amdgpu_waves_per_eu(min_
         }
         Lds<T, PL>::template read<0>(lds, v, t);
         run_phases<T, PL, 0, true>(lds, v, mats + static_cast<size_t>(e) * PL::kD, stride, coef, t);
-
-        T *dst = out[e];
-        if (dst != cur)
-        {
-            if constexpr (ORDERED)
-                flush_piece<T, PL>(cur, acc, t, out, ws, nb, ps, e);
-            else
-                flush<T, PL>(cur, acc, t);
-            cur = dst;
-            ps = e;
-        }
+        if constexpr (!kCoefLds) flush_if_new(e);
 #pragma unroll
         for (int j = 0; j < PL::R; ++j) acc[j] += v[j];
     }

@@ -36,6 +36,7 @@
 #include "kron_split.hpp"
 #include "kron_split_reg.hpp"
 #include "kron_reg.hpp"
+#include "kron_m8.hpp"
 #include "kron_reg_adjoint.hpp"
 #include "kronmult_hip.h"
 
@@ -259,6 +260,14 @@ constexpr int reg_rho(int d, int n)
          : (d == 4 && n == 8) ? 1 : (d == 3 && n == 8) ? 1 : 0;
 }
 
+// n = 8 FP64 plans run on the matrix-core sweep (kron_m8.hpp): the vector sweep feeds each FMA one factor
+// coefficient through an LDS broadcast and is LDS-bound there.
+template<typename T>
+constexpr bool m8_plan(int d, int n)
+{
+    return sizeof(T) == 8 && n == 8 && (d == 3 || d == 4);
+}
+
 // Number of compute units of the current device (cached per device).
 int device_cu_count()
 {
@@ -328,9 +337,17 @@ hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T 
     using PL = kronplan::Plan<D, N, reg_rho(D, N)>;
     static_assert(PL::P % 64 == 0 && PL::P <= 1024, "plan must fill whole wavefronts");
     auto kernel = s.ordered ? kronreg::kron_reg_kernel<T, PL, true> : kronreg::kron_reg_kernel<T, PL, false>;
+    L.lds = sizeof(T) * (PL::SIZE + (kronreg::fwd_coef_in_lds<T, PL>() ? kronreg::Coef<T, PL>::kCount : 0));
+    if constexpr (m8_plan<T>(D, N))
+    {
+        // n = 8 FP64: the matrix-core sweep (kron_m8.hpp), same threads per entry and run structure
+        using MP = kronm8::M8Plan<D>;
+        static_assert(MP::P == PL::P, "one thread per fiber of eight values in both plans");
+        kernel = s.ordered ? kronm8::kron_m8_kernel<D, true> : kronm8::kron_m8_kernel<D, false>;
+        L.lds = sizeof(double) * kronm8::lds_doubles<D>();
+    }
     L.family = KRONMULT_FAMILY_REGISTER;
     L.block = PL::P;
-    L.lds = sizeof(T) * (PL::SIZE + (kronreg::fwd_coef_in_lds<T, PL>() ? kronreg::Coef<T, PL>::kCount : 0));
     L.resident = resident_per_cu(kernel, PL::P, 0);
     // by default no more workgroups than fit at once: a second round of workgroups starts only as the first
     // drains (measured on the d=4 n=8 FP32 batch: 3 resident, 118 us at 3 per CU against 149 us at 4)
@@ -861,7 +878,9 @@ char const *kernel_name(int d, int n, int nb)
     {
     case KRONMULT_FAMILY_MATRIX: return "mfma_sweep_f64";
     case KRONMULT_FAMILY_PACKED: return f64 ? "packed_f64" : "packed_f32";
-    case KRONMULT_FAMILY_REGISTER: return f64 ? "reg_sweep_f64" : "reg_sweep_f32";
+    case KRONMULT_FAMILY_REGISTER:
+        if (m8_plan<T>(s.d, s.n)) return "reg_sweep_mfma_f64";
+        return f64 ? "reg_sweep_f64" : "reg_sweep_f32";
     case KRONMULT_FAMILY_FIBER:
         if (s.n > kMaxFiber)
         {

@@ -8,6 +8,11 @@
 //      groups of 256 bytes (the 32-bank count is printed for reference);
 //   4. the phase schedule, emulated thread by thread with fused multiply-adds, reproduces the oracle's
 //      sweep (oracle/liboracle.so) bit for bit.
+// and for the matrix-core n = 8 FP64 plans (kronmult993_amd/csrc/kron_m8_plan.hpp): layouts and
+// swizzles are bijections, thread and register parts are disjoint bit fields, every LDS access meets the
+// bank model's minimum, and the sweep emulated with the measured semantics of V_MFMA_F64_4X4X4_4B_F64
+// (lane 16 kb + 4 jb + i supplies A_jb[i][kb], lane 16 kb + 4 jb + j supplies B_jb[kb][j], the result in
+// lane 16 i + 4 jb + j, a fused chain over kb from the accumulator) reproduces the oracle bit for bit.
 // Exit status 0 when everything holds; the pytest wrapper runs it on the CPU.
 #include <cmath>
 #include <cstdint>
@@ -17,6 +22,7 @@
 #include <set>
 #include <vector>
 
+#include "kron_m8_plan.hpp"
 #include "kron_plan.hpp"
 
 extern "C" int oracle_kronmult_batched_f64(int d, int n, double const *const *mats, int stride,
@@ -278,8 +284,160 @@ void check(char const *name, bool f64_conflict_free, bool f32_conflict_free)
     check_plan<Plan<D, N, RHO>, float>(name, f32_conflict_free);
 }
 
+// ---- matrix-core n = 8 FP64 plans -----------------------------------------------------------------------
+// One contraction step on the mode whose low index bits are in L4L5 and high bit in register bit KG, the
+// data as MFMA operand A and the factor as operand B (B[kb][j] = a[r = j + 4 rg][k = kb + 4 kg]).
+static void m8_step(std::vector<double> &regs, int P, int kg, double const *A, int stride)
+{
+    std::vector<double> out(regs.size());
+    for (int w = 0; w < P / 64; ++w)
+        for (int f = 0; f < 4; ++f)
+        {
+            // register index from (kg bit value, the two other register bits)
+            int const o0 = kg == 0 ? 1 : 0, o1 = kg == 2 ? 1 : 2;
+            auto reg = [&](int g) { return (g << kg) | ((f & 1) << o0) | (((f >> 1) & 1) << o1); };
+            for (int rg = 0; rg < 2; ++rg)
+                for (int lane = 0; lane < 64; ++lane)
+                {
+                    int const i = lane >> 4, jb = (lane >> 2) & 3, j = lane & 3;  // D_jb[i][j] lands here
+                    double c = 0.0;
+                    for (int g = 0; g < 2; ++g)  // two chained instructions, k = kb + 4 g
+                        for (int kb = 0; kb < 4; ++kb)
+                        {
+                            double const a = regs[static_cast<size_t>(64 * w + 16 * kb + 4 * jb + i) * 8 + reg(g)];
+                            double const b = A[(j + 4 * rg) + (kb + 4 * g) * stride];
+                            c = std::fma(a, b, c);
+                        }
+                    out[static_cast<size_t>(64 * w + lane) * 8 + reg(rg)] = c;
+                }
+        }
+    regs = out;
+}
+
+template<int D>
+void check_m8(char const *name)
+{
+    using PL = kronm8::M8Plan<D>;
+    constexpr int S = PL::SIZE, P = PL::P;
+    std::printf("%s: P=%d R=8 (matrix-core, FP64)\n", name, P);
+    // 1. layouts: bijections, thread and register parts disjoint
+    for (int st = 0; st < 3; ++st)
+    {
+        std::vector<int> seen(S, 0);
+        bool disjoint = true;
+        for (int t = 0; t < P; ++t)
+            for (int j = 0; j < 8; ++j)
+            {
+                ++seen[PL::elem(st, t, j)];
+                disjoint = disjoint && (PL::thread_part(st, t) & PL::reg_part(st, j)) == 0;
+            }
+        bool bij = true;
+        for (int v : seen) bij = bij && v == 1;
+        CHECK(bij, "%s layout %d is not a bijection\n", name, st);
+        CHECK(disjoint, "%s layout %d: thread and register parts overlap\n", name, st);
+    }
+    // 2. swizzles: bijections and GF(2)-linear (phys(a | b) = phys(a) ^ phys(b) for disjoint a, b)
+    {
+        std::vector<int> s0(S, 0), s1(S, 0);
+        bool lin = true;
+        for (int e = 0; e < S; ++e)
+        {
+            ++s0[kronm8::SwizzleIn::phys(e)];
+            ++s1[kronm8::SwizzleEx::phys(e)];
+        }
+        for (int st = 0; st < 3; ++st)
+            for (int t = 0; t < P; ++t)
+                for (int j = 0; j < 8; ++j)
+                {
+                    int const a = PL::thread_part(st, t), b = PL::reg_part(st, j);
+                    lin = lin && kronm8::SwizzleIn::phys(a | b) == (kronm8::SwizzleIn::phys(a) ^ kronm8::SwizzleIn::phys(b));
+                    lin = lin && kronm8::SwizzleEx::phys(a | b) == (kronm8::SwizzleEx::phys(a) ^ kronm8::SwizzleEx::phys(b));
+                }
+        bool bij = true;
+        for (int e = 0; e < S; ++e) bij = bij && s0[e] == 1 && s1[e] == 1;
+        CHECK(bij, "%s swizzle is not a bijection\n", name);
+        CHECK(lin, "%s swizzle is not linear over the layout fields\n", name);
+    }
+    // 3. bank model, per wavefront: input store (b128), P0 read (b64), P0 write (b64), P1 read (b64),
+    // factor reads (b64); the P0 write's minimum is two cycles per 16-lane group (half-rows)
+    {
+        int cyc = 0, ideal = 0;
+        auto run = [&](std::vector<long> const &a, int bytes, bool write, int min_per_group) {
+            cyc += access_cycles(a, bytes, write);
+            int const lanes_per_group = (write ? 128 : 256) / bytes;
+            ideal += static_cast<int>((a.size() + lanes_per_group - 1) / lanes_per_group) * min_per_group;
+        };
+        for (int w = 0; w < P / 64; ++w)
+        {
+            for (int c = 0; c < 4; ++c)
+            {
+                std::vector<long> a;
+                for (int l = 0; l < 64; ++l) a.push_back(8L * kronm8::SwizzleIn::phys(2 * (c * P + 64 * w + l)));
+                run(a, 16, true, 1);
+            }
+            for (int j = 0; j < 8; ++j)
+            {
+                std::vector<long> r0, w0, r1;
+                for (int l = 0; l < 64; ++l)
+                {
+                    r0.push_back(8L * kronm8::SwizzleIn::phys(PL::elem(kronm8::kP0, 64 * w + l, j)));
+                    w0.push_back(8L * kronm8::SwizzleEx::phys(PL::elem(kronm8::kP0, 64 * w + l, j)));
+                    r1.push_back(8L * kronm8::SwizzleEx::phys(PL::elem(kronm8::kP1, 64 * w + l, j)));
+                }
+                run(r0, 8, false, 1);
+                run(w0, 8, true, 2);
+                run(r1, 8, false, 1);
+            }
+            for (int q = 0; q < 2; ++q)  // factor values of one mode: two 16-byte reads of the lane's slot
+            {
+                std::vector<long> a;
+                for (int l = 0; l < 64; ++l) a.push_back(8L * (((l >> 4) & 3) * 16 + (l & 3) * 4 + 2 * q));
+                run(a, 16, false, 1);
+            }
+        }
+        std::printf("    LDS: %d cycles for a minimum of %d\n", cyc, ideal);
+        CHECK(cyc == ideal, "%s has LDS bank conflicts beyond the layout's minimum\n", name);
+    }
+    // 4. emulate the sweep and compare with the oracle, bit for bit
+    std::mt19937_64 rng(77 + D);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    int const stride = 11;
+    std::vector<double> A(static_cast<size_t>(D) * 8 * stride);
+    for (auto &a : A) a = U(rng);
+    std::vector<double> x(S);
+    for (auto &v : x) v = U(rng);
+    std::vector<double const *> mats(D);
+    for (int t = 0; t < D; ++t) mats[t] = A.data() + static_cast<size_t>(t) * 8 * stride;
+    std::vector<double> regs(static_cast<size_t>(P) * 8), tensor(S);
+    for (int t = 0; t < P; ++t)
+        for (int j = 0; j < 8; ++j) regs[static_cast<size_t>(t) * 8 + j] = x[PL::elem(kronm8::kP0, t, j)];
+    m8_step(regs, P, 0, mats[D - 1], stride);
+    m8_step(regs, P, 1, mats[D - 2], stride);
+    for (int t = 0; t < P; ++t)  // the exchange: P0 out, P1 in
+        for (int j = 0; j < 8; ++j) tensor[PL::elem(kronm8::kP0, t, j)] = regs[static_cast<size_t>(t) * 8 + j];
+    for (int t = 0; t < P; ++t)
+        for (int j = 0; j < 8; ++j) regs[static_cast<size_t>(t) * 8 + j] = tensor[PL::elem(kronm8::kP1, t, j)];
+    if (D == 3)
+        m8_step(regs, P, 0, mats[0], stride);
+    else
+    {
+        m8_step(regs, P, 0, mats[1], stride);
+        m8_step(regs, P, 1, mats[0], stride);
+    }
+    for (int t = 0; t < P; ++t)
+        for (int j = 0; j < 8; ++j) tensor[PL::elem(kronm8::kF, t, j)] = regs[static_cast<size_t>(t) * 8 + j];
+    std::vector<double> ref(S, 0.0);
+    double const *inp = x.data();
+    double *outp = ref.data();
+    oracle_kronmult_batched_f64(D, 8, mats.data(), stride, &inp, &outp, 1, 1);
+    CHECK(std::memcmp(ref.data(), tensor.data(), sizeof(double) * S) == 0, "%s emulation differs from the oracle\n",
+          name);
+}
+
 int main()
 {
+    check_m8<3>("m8 d3");
+    check_m8<4>("m8 d4");
     check<6, 4, 2>("d6n4r2", true, true);
     check<6, 4, 3>("d6n4r3", false, false);  // not instantiated by the library
     check<4, 8, 1>("d4n8r1", true, true);
