@@ -136,16 +136,14 @@ __device__ __forceinline__ void flush_piece(double *dst_generic, double (&acc)[8
     }
 }
 
-#ifndef KRON_M8_TWO_BUFFERS
-#define KRON_M8_TWO_BUFFERS 1
-#endif
 // d = 4 (eight wavefronts) keeps the staged input and the exchange in two LDS buffers, and the factors in
 // two alternating ones: the next entry's stores then need no barrier of their own (the last readers of
-// each buffer are ordered before them by the barriers in between), two barriers per entry instead of four.
+// each buffer are ordered before them by the barriers in between), two barriers per entry instead of four
+// (d=4 n=8 batch: 142.4 against 145.5 us per launch).
 template<int D>
 constexpr bool two_buffers()
 {
-    return D == 4 && KRON_M8_TWO_BUFFERS;
+    return D == 4;
 }
 
 template<int D>
