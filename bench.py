@@ -61,6 +61,8 @@ WORKLOADS = {
                   "N(0,1) data; multi-pass sweep through the workspace"),
     "c2_terms": ("the d=6 n=4 operator with 3 Kronecker terms per element pair (each term its own six 64x64 "
                  "coefficient matrices), 8192 entries x 3 terms, 128 outputs; with --op terms"),
+    "c2_input": ("the d=6 n=4 batch's 8192 entries in input-major order (the 128 entries of one input "
+                 "consecutive): the adjoint's order, d(input) summed per run of equal input pointers"),
 }
 
 

@@ -156,8 +156,11 @@ struct Adj
         }
     }
 
-    // dA_m from the fiber-major H (c.ht) and X (xfm); ends with a barrier-free atomic flush
-    static __device__ __forceinline__ void factor_gradient(Ctx const &c, T const *xfm, int m)
+    // dA_m from the fiber-major H (c.ht) and X (xfm), added to this thread's partial gacc; with flush, the
+    // partials are reduced across the workgroup and added to the gradient (one atomic per element) -- the
+    // caller flushes when the next entry's gradient pointer for this mode differs, so entries that share a
+    // factor gradient are summed on chip first
+    static __device__ __forceinline__ void factor_gradient(Ctx const &c, T const *xfm, int m, T &gacc, bool flush)
     {
         using V = typename kronreg::Granule<T>::type;
         constexpr int G = S::G;
@@ -190,6 +193,10 @@ struct Adj
         for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int i = 0; i < G; ++i) acc += part[u][i];
+        gacc += acc;
+        if (!flush) return;
+        acc = gacc;
+        gacc = T(0);
         if constexpr (S::NSUB > 1)
         {
             c.red[sub * S::NN + o] = acc;
@@ -214,7 +221,8 @@ struct Adj
     // across lanes afterwards. Wavefront w takes fibers [w*FW, (w+1)*FW); partial results meet in LDS.
     static constexpr bool kMfmaGradient = sizeof(T) == 8 && (N == 4 || N == 8);
 
-    static __device__ __forceinline__ void factor_gradient_mfma(Ctx const &c, T const *xfm, int m)
+    static __device__ __forceinline__ void factor_gradient_mfma(Ctx const &c, T const *xfm, int m, T &gacc,
+                                                                bool flush)
     {
         constexpr int NW = P / 64;
         constexpr int FW = S::NFIB / NW;
@@ -251,7 +259,10 @@ struct Adj
             r = kb;
             col = lo;
         }
-        double acc = acc0 + acc1;
+        gacc += acc0 + acc1;  // the shuffles and the cross-wave sum below are linear: applied at the flush
+        if (!flush) return;
+        double acc = gacc;
+        gacc = 0.0;
         if constexpr (N == 4)
         {
             acc += __shfl_xor(acc, 4);
@@ -279,7 +290,7 @@ struct Adj
     // xk[d-1] = x in every mode)
     template<int TT, bool STORE, bool KEEP>
     static __device__ __forceinline__ void hstep(Ctx const &c, T (&h)[R], T const (&xk)[D][R], bool want_mats,
-                                                 bool want_input)
+                                                 bool want_input, T (&gacc)[D], unsigned flush_mask)
     {
         constexpr int PH = D - 1 - TT;  // phase with mode TT in registers
         if (want_mats)
@@ -311,16 +322,17 @@ struct Adj
                 xfm = c.ex;
             }
             lds_barrier();
+            bool const flush = (flush_mask >> TT) & 1u;
             if constexpr (kMfmaGradient)
-                factor_gradient_mfma(c, xfm, TT);
+                factor_gradient_mfma(c, xfm, TT, gacc[TT], flush);
             else
-                factor_gradient(c, xfm, TT);
+                factor_gradient(c, xfm, TT, gacc[TT], flush);
         }
         if (TT < D - 1 || want_input) contract<T, PL, true>(h, c.A[TT], c.stride, coef_of(c, TT));
         if constexpr (TT < D - 1)
         {
             exchange<PH, PH - 1>(c, h);
-            hstep<TT + 1, STORE, KEEP>(c, h, xk, want_mats, want_input);
+            hstep<TT + 1, STORE, KEEP>(c, h, xk, want_mats, want_input, gacc, flush_mask);
         }
     }
 
@@ -409,8 +421,32 @@ __global__ void __launch_bounds__(PL::P)
     };
     prefetch(e0);
 
+    // run sums (n = 4 plans): factor-gradient partials per mode while consecutive entries share that
+    // gradient, d x while they share the input gradient (an input-major batch then adds each input's
+    // gradient once per run). The n = 8 plans flush every entry: their registers are spent (the d=3 n=8
+    // FP64 kernel would spill under its three-waves cap), and their batches rarely repeat factors in a row.
+    constexpr bool kRun = PL::kN == 4;
+    T gacc[D], dxacc[R];
+#pragma unroll
+    for (int m = 0; m < D; ++m) gacc[m] = T(0);
+#pragma unroll
+    for (int j = 0; j < R; ++j) dxacc[j] = T(0);
+
     for (int e = e0; e < e1; ++e)
     {
+        bool const last = e + 1 >= e1;
+        unsigned flush_mask = ~0u;
+        if constexpr (kRun)
+        {
+            flush_mask = 0;
+            if (want_mats)
+            {
+#pragma unroll
+                for (int m = 0; m < D; ++m)
+                    if (last || gmats[static_cast<size_t>(e + 1) * D + m] != gmats[static_cast<size_t>(e) * D + m])
+                        flush_mask |= 1u << m;
+            }
+        }
         T xk[D][R], h[R];  // xk[d-1]: x in the phase-0 layout
 #pragma unroll
         for (int j = 0; j < R; ++j)
@@ -438,15 +474,24 @@ __global__ void __launch_bounds__(PL::P)
             }
         }
         lds_barrier();  // staged factors (and stored chain) visible
-        A::template hstep<0, STORE, KEEP>(c, h, xk, want_mats, want_input);
+        A::template hstep<0, STORE, KEEP>(c, h, xk, want_mats, want_input, gacc, flush_mask);
         if (want_input)
         {
-            // h = d x in the phase-0 layout (elements t*R + j): through LDS into the phase-(d-1) layout
-            // (elements j*P + t), so that every atomic instruction covers one contiguous stretch
-            A::template exchange<0, D - 1>(c, h);
-            global_ptr<T> gi = as_global(gin[e]);
 #pragma unroll
-            for (int j = 0; j < R; ++j) atomicAdd((T *)(gi + j * P + t), h[j]);
+            for (int j = 0; j < R; ++j) dxacc[j] += h[j];
+            if (!kRun || last || gin[e + 1] != gin[e])
+            {
+                // d x in the phase-0 layout (elements t*R + j): through LDS into the phase-(d-1) layout
+                // (elements j*P + t), so that every atomic instruction covers one contiguous stretch
+                A::template exchange<0, D - 1>(c, dxacc);
+                global_ptr<T> gi = as_global(gin[e]);
+#pragma unroll
+                for (int j = 0; j < R; ++j)
+                {
+                    atomicAdd((T *)(gi + j * P + t), dxacc[j]);
+                    dxacc[j] = T(0);
+                }
+            }
         }
     }
 }

@@ -120,27 +120,36 @@ def c1(seed: int = 0, nb: int = 4096) -> HostBatch:
                      (nb + 7) // 8, np.arange(nb, dtype=np.int64) // 8)
 
 
-def c2(seed: int = 0, rows: int = 128, cols: int = 64) -> HostBatch:
+def c2(seed: int = 0, rows: int = 128, cols: int = 64, order: str = "output") -> HostBatch:
     """ASGarD-like 6D operator apply, d=6, n=4, stride 64.
 
     For each dimension t a 64x64 column-major coefficient matrix C_t ~ N(0,1)/8. Entry (i, j) (row element
-    i < rows, column element j < cols, k = i*cols + j) uses, in dimension t, the 4x4 block of C_t at
-    (4*r, 4*c) where r and c are digit t of i and j in base 16 (digit t = floor(id / 16^t) mod 16).
-    Entry (i, j) reads input[j] (4096 values U[-1,1]) and accumulates into output[i].
+    i < rows, column element j < cols) uses, in dimension t, the 4x4 block of C_t at (4*r, 4*c) where r and
+    c are digit t of i and j in base 16 (digit t = floor(id / 16^t) mod 16). Entry (i, j) reads input[j]
+    (4096 values U[-1,1]) and accumulates into output[i]. order "output": entry k = i*cols + j (the
+    forward's order, entries of one output consecutive); "input": k = j*rows + i (the same entries with
+    those of one input consecutive, the adjoint's order). Same data either way.
     """
+    if order not in ("output", "input"):
+        raise ValueError("order is 'output' or 'input'")
     d, n, ld = 6, 4, 64
     size = n ** d
     coef = np.concatenate([gen.normal(seed, 10 + t, ld * ld) / 8.0 for t in range(d)])
-    i = np.repeat(np.arange(rows, dtype=np.int64), cols)
-    j = np.tile(np.arange(cols, dtype=np.int64), rows)
+    if order == "output":
+        i = np.repeat(np.arange(rows, dtype=np.int64), cols)
+        j = np.tile(np.arange(cols, dtype=np.int64), rows)
+    else:
+        j = np.repeat(np.arange(cols, dtype=np.int64), rows)
+        i = np.tile(np.arange(rows, dtype=np.int64), cols)
     moff = np.empty((rows * cols, d), dtype=np.int64)
     for t in range(d):
         r = (i >> (4 * t)) & 15
         c = (j >> (4 * t)) & 15
         moff[:, t] = t * ld * ld + 4 * r + 4 * c * ld
     inp = gen.uniform(seed, 2, cols * size)
-    return HostBatch("c2_d6n4_asgard", d, n, ld, coef, moff.reshape(-1), inp, j * size, rows, i,
-                     meta={"rows": rows, "cols": cols})
+    name = "c2_d6n4_asgard" if order == "output" else "c2_d6n4_asgard_input_major"
+    return HostBatch(name, d, n, ld, coef, moff.reshape(-1), inp, j * size, rows, i,
+                     meta={"rows": rows, "cols": cols, "order": order})
 
 
 def expand_terms(b: HostBatch, factor_pools) -> HostBatch:
@@ -277,5 +286,6 @@ def ref_large(seed: int = 0) -> HostBatch:
 
 
 def by_name(name: str, seed: int = 0):
-    table = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5, "ref_large": ref_large, "c2_terms": c2_terms}
+    table = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5, "ref_large": ref_large, "c2_terms": c2_terms,
+             "c2_input": lambda seed: c2(seed=seed, order="input")}
     return table[name](seed=seed)

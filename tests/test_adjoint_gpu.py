@@ -104,6 +104,20 @@ def test_adjoint_c5_f32(gpu):
     _check(b, gi, gm)
 
 
+@pytest.mark.parametrize("order", ["output", "input"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_adjoint_c2_run_sums(gpu, order, dtype):
+    """d=6 n=4 operator subset (24 output x 16 input elements, 384 entries, factor blocks of modes 2..5 shared
+    by every entry): in input-major order the kernel sums d(input) over each run of 24 entries with one
+    input before its atomics, and in both orders the factor-gradient partials over the entries that share a
+    gradient pointer in a row; both must equal the oracle adjoint."""
+    from kronmult993_amd import gen
+    b = W.c2(rows=24, cols=16, order=order).astype(dtype)
+    b.grad_out_pool = gen.normal(0, 90, b.out_count * b.size).astype(dtype)
+    gi, gm = _gpu_adjoint(b, gpu, "auto")
+    _check(b, gi, gm)
+
+
 def test_adjoint_invalid(gpu):
     import torch
     b = W.random_case(seed=1, d=2, n=9, nb=3)  # n > 8 is not supported by the adjoint
