@@ -983,8 +983,11 @@ hipError_t reg_adjoint_launch(Shape const &s, RegAdjMode mode, T const *const *m
 {
     using PL = kronplan::Plan<D, N, 1>;
     // short runs (about one entry per workgroup at the d=3 n=8 batch size): the adjoint's per-entry chain is
-    // long, so balance beats prefetch depth (measured: 46 us vs 59 us with the forward's run length)
-    int const per_cu = 4 * default_workgroups_per_cu(PL::P);
+    // long, so balance beats prefetch depth (measured: 46 us vs 59 us with the forward's run length). The
+    // d=6 n=4 plan (1024 threads, one workgroup per CU) takes one run per CU instead: its run sums of the
+    // input and factor gradients then cover 32 entries of the 8192-entry batch (FP64 569 -> 521 us, input-
+    // major 562 -> 487 us; FP32 347 -> 304 us; two or four runs per CU: 524 / 533 us)
+    int const per_cu = PL::P >= 1024 ? 1 : 4 * default_workgroups_per_cu(PL::P);
     long const slots = static_cast<long>(device_cu_count()) * per_cu;
     int epw = static_cast<int>((s.nb + slots - 1) / slots);
     if (epw < 1) epw = 1;
