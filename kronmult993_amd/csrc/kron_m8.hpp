@@ -139,7 +139,8 @@ __device__ __forceinline__ void flush_piece(double *dst_generic, double (&acc)[8
 // d = 4 (eight wavefronts) keeps the staged input and the exchange in two LDS buffers, and the factors in
 // two alternating ones: the next entry's stores then need no barrier of their own (the last readers of
 // each buffer are ordered before them by the barriers in between), two barriers per entry instead of four
-// (d=4 n=8 batch: 142.4 against 145.5 us per launch).
+// (d=4 n=8 batch: 142.4 against 145.5 us per launch). A single buffer under an 80-register cap, three
+// workgroups per CU, measured slower (152-158 against 140-148 us).
 template<int D>
 constexpr bool two_buffers()
 {
