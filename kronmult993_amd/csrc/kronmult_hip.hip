@@ -346,13 +346,14 @@ hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T 
         kernel = s.ordered ? kronm8::kron_m8_kernel<D, true> : kronm8::kron_m8_kernel<D, false>;
         L.lds = sizeof(double) * kronm8::lds_doubles<D>();
     }
+    int const block = PL::P;
     L.family = KRONMULT_FAMILY_REGISTER;
-    L.block = PL::P;
-    L.resident = resident_per_cu(kernel, PL::P, 0);
+    L.block = block;
+    L.resident = resident_per_cu(kernel, block, 0);
     // by default no more workgroups than fit at once: a second round of workgroups starts only as the first
     // drains (measured on the d=4 n=8 FP32 batch: 3 resident, 118 us at 3 per CU against 149 us at 4)
     int const per_cu = tu.workgroups_per_cu > 0 ? tu.workgroups_per_cu
-                                                : std::min(default_workgroups_per_cu(PL::P), L.resident);
+                                                : std::min(default_workgroups_per_cu(block), L.resident);
     long const slots = static_cast<long>(device_cu_count()) * per_cu;
     int epw = tu.entries_per_workgroup > 0 ? tu.entries_per_workgroup : static_cast<int>((s.nb + slots - 1) / slots);
     if (epw < 1) epw = 1;
@@ -360,7 +361,7 @@ hipError_t reg_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T 
     long const runs = (s.nb + epw - 1) / epw;
     L.grid = (runs + 7) / 8 * 8;  // surplus runs exit at once
     if (!run) return hipSuccess;
-    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(L.grid)), dim3(PL::P), 0, stream, s.nb, epw, mats,
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(L.grid)), dim3(block), 0, stream, s.nb, epw, mats,
                        s.stride, in, out, ws);
     return hipGetLastError();
 }

@@ -62,7 +62,7 @@ def test_ordered_register_mixed_alignment(gpu, epw):
 
 
 @pytest.mark.parametrize("d,n,nb,dtype", [(3, 4, 300, np.float64), (3, 4, 300, np.float32), (5, 3, 90, np.float64),
-                                          (4, 8, 70, np.float64), (2, 20, 40, np.float64)])
+                                          (4, 8, 70, np.float64), (3, 8, 70, np.float64), (2, 20, 40, np.float64)])
 def test_ordered_irregular_batches(gpu, d, n, nb, dtype):
     """Scattered aliasing regrouped by output, padded strides, shared factors, non-zero initial outputs."""
     b = grouped_by_output(W.random_case(90 + d + n, d, n, nb, shared_factors=7, dtype=dtype))
