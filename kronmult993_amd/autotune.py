@@ -63,8 +63,6 @@ def entries_per_pass(d: int, n: int, itemsize: int) -> int:
     e = max(1, GROUP_THREADS // nfib)
     while e > 1 and e * per_entry > LDS_BUDGET:
         e -= 1
-    while e > 1 and e * (d + 1) > 2 * GROUP_THREADS:   # a pass's pointers: two per thread
-        e -= 1
     return e
 
 

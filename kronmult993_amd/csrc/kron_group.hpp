@@ -69,34 +69,9 @@ __host__ __device__ constexpr int ptr_elems(int d)
     return static_cast<int>(((d + 1) * sizeof(void *) + sizeof(T) - 1) / sizeof(T));
 }
 
-// The pointers of one pass (cnt input pointers, then cnt * d factor pointers: at most 2 * kThreads, since
-// a pass holds at most kThreads / n^(d-1) entries; and cnt output pointers), one slot per thread and
-// register, loaded a pass ahead so that the pass starts with its data loads.
-template<typename T>
-struct PassPtrs
-{
-    T const *p0, *p1;
-    T *po;
-
-    __device__ __forceinline__ void load(GroupDesc<T> const &G, int e0, int cnt, int tid)
-    {
-        int const nptr = cnt * (G.d + 1);
-        auto at = [&](int j) -> T const * {
-            if (j < cnt) return G.in[e0 + j];
-            if (j < nptr) return G.mats[static_cast<size_t>(e0) * G.d + (j - cnt)];
-            return nullptr;
-        };
-        p0 = at(tid);
-        p1 = at(tid + kThreads);
-        po = tid < cnt ? G.out[e0 + tid] : nullptr;
-    }
-};
-
-// process entries [e0, e0 + cnt) of group G (cnt <= G.E) with N = G.n fixed at compile time; pp holds this
-// thread's slots of the pass's pointers
+// process entries [e0, e0 + cnt) of group G (cnt <= G.E) with N = G.n fixed at compile time
 template<typename T, int N>
-__device__ __forceinline__ void pass(GroupDesc<T> const &G, int e0, int cnt, T *lds, T **out_slot,
-                                     PassPtrs<T> const &pp)
+__device__ __forceinline__ void pass(GroupDesc<T> const &G, int e0, int cnt, T *lds, T **out_slot)
 {
     int const d = G.d, size = G.size, nfib = G.nfib;
     int const nn = N * N;
@@ -111,9 +86,9 @@ __device__ __forceinline__ void pass(GroupDesc<T> const &G, int e0, int cnt, T *
     int const tid = threadIdx.x;
 
     int const nptr = cnt * (d + 1);
-    if (tid < nptr) ptr[tid] = pp.p0;
-    if (tid + kThreads < nptr) ptr[tid + kThreads] = pp.p1;
-    if (tid < cnt) out_slot[tid] = pp.po;  // the pass's output pointers
+    for (int j = tid; j < nptr; j += blockDim.x)
+        ptr[j] = j < cnt ? G.in[e0 + j] : G.mats[static_cast<size_t>(e0) * d + (j - cnt)];
+    if (tid < cnt) out_slot[tid] = G.out[e0 + tid];  // the pass's output pointers
     __syncthreads();
     for (int i = tid; i < cnt * size; i += blockDim.x)
     {
@@ -177,50 +152,29 @@ __global__ void __launch_bounds__(kThreads) kron_group_kernel(GroupTable<T> cons
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ T *out_slot[kThreads];  // a pass holds at most kThreads entries (one fiber each at least)
     T *const lds = reinterpret_cast<T *>(smem_raw);
-    int const tid = threadIdx.x;
-    // this workgroup's passes in order: tiles blockIdx.x, + gridDim.x, ...; passes of E entries in a tile
-    struct Pos
+    for (int tile = blockIdx.x; tile < table.tiles; tile += gridDim.x)
     {
-        int tile, gi, e0, e_end;
-    };
-    auto tile_pos = [&](int tile) {
-        Pos p{tile, 0, 0, 0};
-        if (tile >= table.tiles) return p;
-        while (p.gi + 1 < table.count && tile >= table.tile_start[p.gi + 1]) ++p.gi;
-        GroupDesc<T> const &G = table.g[p.gi];
-        p.e0 = (tile - table.tile_start[p.gi]) * G.epw;
-        p.e_end = p.e0 + G.epw < G.nb ? p.e0 + G.epw : G.nb;
-        return p;
-    };
-    auto count = [&](Pos const &p) {
-        int const E = table.g[p.gi].E;
-        return p.e_end - p.e0 < E ? p.e_end - p.e0 : E;
-    };
-    Pos cur = tile_pos(blockIdx.x);
-    PassPtrs<T> pp, pn;
-    if (cur.tile < table.tiles) pp.load(table.g[cur.gi], cur.e0, count(cur), tid);
-    while (cur.tile < table.tiles)
-    {
-        GroupDesc<T> const &G = table.g[cur.gi];
-        int const cnt = count(cur);
-        Pos nxt = cur;
-        nxt.e0 += cnt;
-        if (nxt.e0 >= nxt.e_end) nxt = tile_pos(cur.tile + gridDim.x);
-        if (nxt.tile < table.tiles) pn.load(table.g[nxt.gi], nxt.e0, count(nxt), tid);
-        switch (G.n)
+        int gi = 0;
+        while (gi + 1 < table.count && tile >= table.tile_start[gi + 1]) ++gi;
+        GroupDesc<T> const &G = table.g[gi];
+        int const e_begin = (tile - table.tile_start[gi]) * G.epw;
+        int const e_end = e_begin + G.epw < G.nb ? e_begin + G.epw : G.nb;
+        for (int e0 = e_begin; e0 < e_end; e0 += G.E)
         {
-        case 1: pass<T, 1>(G, cur.e0, cnt, lds, out_slot, pp); break;
-        case 2: pass<T, 2>(G, cur.e0, cnt, lds, out_slot, pp); break;
-        case 3: pass<T, 3>(G, cur.e0, cnt, lds, out_slot, pp); break;
-        case 4: pass<T, 4>(G, cur.e0, cnt, lds, out_slot, pp); break;
-        case 5: pass<T, 5>(G, cur.e0, cnt, lds, out_slot, pp); break;
-        case 6: pass<T, 6>(G, cur.e0, cnt, lds, out_slot, pp); break;
-        case 7: pass<T, 7>(G, cur.e0, cnt, lds, out_slot, pp); break;
-        case 8: pass<T, 8>(G, cur.e0, cnt, lds, out_slot, pp); break;
-        default: break;  // rejected on the host
+            int const cnt = e_end - e0 < G.E ? e_end - e0 : G.E;
+            switch (G.n)
+            {
+            case 1: pass<T, 1>(G, e0, cnt, lds, out_slot); break;
+            case 2: pass<T, 2>(G, e0, cnt, lds, out_slot); break;
+            case 3: pass<T, 3>(G, e0, cnt, lds, out_slot); break;
+            case 4: pass<T, 4>(G, e0, cnt, lds, out_slot); break;
+            case 5: pass<T, 5>(G, e0, cnt, lds, out_slot); break;
+            case 6: pass<T, 6>(G, e0, cnt, lds, out_slot); break;
+            case 7: pass<T, 7>(G, e0, cnt, lds, out_slot); break;
+            case 8: pass<T, 8>(G, e0, cnt, lds, out_slot); break;
+            default: break;  // rejected on the host
+            }
         }
-        pp = pn;
-        cur = nxt;
     }
 }
 

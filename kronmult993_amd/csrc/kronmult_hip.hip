@@ -679,8 +679,6 @@ size_t fill_group(Shape const &s, T const *const *mats, T const *const *in, T *c
     int E = krongroup::kThreads / D.nfib;
     if (E < 1) E = 1;
     while (E > 1 && E * per_entry > static_cast<long>(kGroupLdsBudget)) --E;
-    // a pass's input and factor pointers fit two per thread (krongroup::PassPtrs)
-    while (E > 1 && E * (s.d + 1) > 2 * krongroup::kThreads) --E;
     // a tile shorter than one pass shrinks the pass (and the LDS the launch needs, so more workgroups fit)
     if (entries_per_tile > 0 && entries_per_tile < E) E = entries_per_tile;
     D.E = E;
