@@ -38,6 +38,7 @@
 #include "kron_reg.hpp"
 #include "kron_m8.hpp"
 #include "kron_reg_adjoint.hpp"
+#include "kron_reg_adjoint2.hpp"
 #include "kronmult_hip.h"
 
 namespace
@@ -1016,10 +1017,32 @@ hipError_t reg_adjoint_launch(Shape const &s, RegAdjMode mode, T const *const *m
     }
 }
 
+// d = 6, n = 4: the two-modes-per-phase adjoint (kron_reg_adjoint2.hpp), one run per resident workgroup
+#ifndef KRON_ADJ2
+#define KRON_ADJ2 1
+#endif
+template<typename T>
+hipError_t adjoint2_launch(Shape const &s, T const *const *mats, T const *const *in, T const *const *gout,
+                           T *const *gin, T *const *gmats, hipStream_t stream)
+{
+    auto kernel = kronregadj2::kron_reg_adjoint2_kernel<T>;
+    int const per_cu = resident_per_cu(kernel, kronregadj2::kP, 0);
+    long const slots = static_cast<long>(device_cu_count()) * per_cu;
+    int epw = static_cast<int>((s.nb + slots - 1) / slots);
+    if (epw < 1) epw = 1;
+    long const runs = (s.nb + epw - 1) / epw;
+    long const grid = (runs + 7) / 8 * 8;
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(grid)), dim3(kronregadj2::kP), 0, stream, s.nb, epw, mats,
+                       s.stride, in, gout, gin, gmats);
+    return hipGetLastError();
+}
+
 template<typename T>
 hipError_t reg_adjoint_dispatch(Shape const &s, int intermediates, T const *const *mats, T const *const *in,
                                 T const *const *gout, T *const *gin, T *const *gmats, hipStream_t stream)
 {
+    if (KRON_ADJ2 && s.d == 6 && s.n == 4 && intermediates == KRONMULT_ADJOINT_AUTO)
+        return adjoint2_launch<T>(s, mats, in, gout, gin, gmats, stream);
     // STORE: the chain in LDS where it fits comfortably (d=3 n=8, d=4 n=4), else in registers.
     // AUTO, measured per shape: d=4 n=8 and d=5/6 n=4 keep the chain in registers (d=6 n=4 FP64 0.58 ms
     // against 0.78 ms recomputing, FP32 0.38 against 0.49; d=4 n=8 FP64 1.01 against 1.15 ms); d=3 n=8

@@ -14,6 +14,7 @@
 // (lane 16 kb + 4 jb + i supplies A_jb[i][kb], lane 16 kb + 4 jb + j supplies B_jb[kb][j], the result in
 // lane 16 i + 4 jb + j, a fused chain over kb from the accumulator) reproduces the oracle bit for bit.
 // Exit status 0 when everything holds; the pytest wrapper runs it on the CPU.
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -27,6 +28,9 @@
 
 extern "C" int oracle_kronmult_batched_f64(int d, int n, double const *const *mats, int stride,
                                            double const *const *in, double *const *out, int nb, int fused);
+extern "C" int oracle_kronmult_adjoint_f64(int d, int n, double const *const *mats, int stride,
+                                           double const *const *in, double const *const *gout, double *const *gin,
+                                           double *const *gmats, int nb, int fused);
 extern "C" int oracle_kronmult_batched_f32(int d, int n, float const *const *mats, int stride,
                                            float const *const *in, float *const *out, int nb, int fused);
 
@@ -434,8 +438,159 @@ void check_m8(char const *name)
           name);
 }
 
+// ---- the d = 6, n = 4 two-modes-per-phase adjoint (kron_reg_adjoint2.hpp), emulated thread by thread and
+// lane by lane (its reduce-scatter), against the oracle adjoint ----------------------------------------------
+static void check_adjoint2()
+{
+    using PL = Plan<6, 4, 2>;
+    constexpr int R = PL::R, P = PL::P, S = PL::SIZE;
+    std::printf("adjoint2 d6n4: P=%d R=%d\n", P, R);
+    auto phase_of = [](int m) { return m >= 4 ? 0 : m >= 2 ? 1 : 2; };
+    auto stride_of = [&](int m) { return PL::reg_stride(m - PL::lo(phase_of(m))); };
+    std::mt19937_64 rng(404);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    int const stride = 7;
+    std::vector<double> A(6 * 4 * stride);
+    for (auto &a : A) a = U(rng);
+    std::vector<double> x(S), g(S);
+    for (auto &v : x) v = U(rng);
+    for (auto &v : g) v = U(rng);
+    auto fac = [&](int m, int r, int k, bool trans) {
+        return trans ? A[m * 4 * stride + k + r * stride] : A[m * 4 * stride + r + k * stride];
+    };
+    using Regs = std::vector<std::array<double, 16>>;
+    auto contract = [&](Regs &v, int m, bool trans) {
+        int const st = stride_of(m);
+        for (int t = 0; t < P; ++t)
+            for (int base = 0; base < R; ++base)
+            {
+                if ((base / st) % 4 != 0) continue;
+                double xk[4];
+                for (int k = 0; k < 4; ++k) xk[k] = v[t][base + k * st];
+                for (int r = 0; r < 4; ++r)
+                {
+                    double acc = 0;
+                    for (int k = 0; k < 4; ++k) acc = std::fma(xk[k], fac(m, r, k, trans), acc);
+                    v[t][base + r * st] = acc;
+                }
+            }
+    };
+    auto exchange = [&](Regs &v, int a, int b) {
+        std::vector<double> ten(S);
+        for (int t = 0; t < P; ++t)
+            for (int j = 0; j < R; ++j) ten[PL::elem(a, t, j)] = v[t][j];
+        for (int t = 0; t < P; ++t)
+            for (int j = 0; j < R; ++j) v[t][j] = ten[PL::elem(b, t, j)];
+    };
+    auto load = [&](Regs &v, std::vector<double> const &src, int ph) {
+        for (int t = 0; t < P; ++t)
+            for (int j = 0; j < R; ++j) v[t][j] = src[PL::elem(ph, t, j)];
+    };
+    double dA[6][16] = {};
+    // the kernel's partial, reduce-scatter and flush, lane by lane
+    auto grad = [&](Regs const &h, Regs const &xv, int m) {
+        int const st = stride_of(m), o = st == 4 ? 1 : 4;
+        std::vector<std::array<double, 16>> p(P);
+        for (int t = 0; t < P; ++t)
+            for (int r = 0; r < 4; ++r)
+                for (int c = 0; c < 4; ++c)
+                {
+                    double acc = 0;
+                    for (int f = 0; f < 4; ++f) acc = std::fma(h[t][r * st + f * o], xv[t][c * st + f * o], acc);
+                    p[t][4 * r + c] = acc;
+                }
+        std::vector<double> val(P);
+        for (int w = 0; w < P / 64; ++w)
+        {
+            std::vector<std::vector<double>> cur(64);
+            for (int l = 0; l < 64; ++l) cur[l].assign(p[64 * w + l].begin(), p[64 * w + l].end());
+            for (int bit = 5, half = 8; bit >= 2; --bit, half /= 2)
+            {
+                std::vector<std::vector<double>> nxt(64, std::vector<double>(half));
+                for (int l = 0; l < 64; ++l)
+                {
+                    int const b = (l >> bit) & 1, pl = l ^ (1 << bit);
+                    for (int k = 0; k < half; ++k)
+                    {
+                        double const mine = b ? cur[l][k + half] : cur[l][k];
+                        int const pb = (pl >> bit) & 1;
+                        double const other_of_partner = pb ? cur[pl][k] : cur[pl][k + half];
+                        nxt[l][k] = mine + other_of_partner;
+                    }
+                }
+                cur = nxt;
+            }
+            for (int l = 0; l < 64; ++l) val[64 * w + l] = cur[l][0];
+        }
+        // flush: quad sums, then the four wavefronts
+        double red[4][16] = {};
+        for (int w = 0; w < P / 64; ++w)
+            for (int l = 0; l < 64; l += 4)
+            {
+                double v = val[64 * w + l];
+                v += val[64 * w + (l ^ 1)];
+                v += val[64 * w + (l ^ 2)] + val[64 * w + (l ^ 3)];
+                int const idx = ((l >> 5) & 1) * 8 + ((l >> 4) & 1) * 4 + ((l >> 3) & 1) * 2 + ((l >> 2) & 1);
+                red[w][idx] = v;
+            }
+        for (int i = 0; i < 16; ++i)
+            for (int w = 0; w < P / 64; ++w) dA[m][i] += red[w][i];
+    };
+    Regs h(P), v(P), xs(P);
+    load(h, g, 2);
+    load(v, x, 0);
+    contract(v, 5, false); contract(v, 4, false); exchange(v, 0, 1);
+    contract(v, 3, false); contract(v, 2, false); exchange(v, 1, 2);
+    xs = v; contract(v, 1, false);
+    grad(h, v, 0);
+    contract(h, 0, true); grad(h, xs, 1);
+    contract(h, 1, true); exchange(h, 2, 1);
+    load(v, x, 0); contract(v, 5, false); contract(v, 4, false); exchange(v, 0, 1);
+    xs = v; contract(v, 3, false);
+    grad(h, v, 2);
+    contract(h, 2, true); grad(h, xs, 3);
+    contract(h, 3, true); exchange(h, 1, 0);
+    load(v, x, 0); xs = v; contract(v, 5, false);
+    grad(h, v, 4);
+    contract(h, 4, true); grad(h, xs, 5);
+    contract(h, 5, true);
+    exchange(h, 0, 2);
+    std::vector<double> dx(S);
+    for (int t = 0; t < P; ++t)
+        for (int j = 0; j < R; ++j) dx[j * P + t] = h[t][j];
+    // oracle
+    std::vector<double> rdx(S, 0.0), rdA(6 * 4 * stride, 0.0);
+    std::vector<double const *> mats(6);
+    std::vector<double *> gm(6);
+    for (int m = 0; m < 6; ++m)
+    {
+        mats[m] = A.data() + m * 4 * stride;
+        gm[m] = rdA.data() + m * 4 * stride;
+    }
+    double const *ip = x.data(), *gp = g.data();
+    double *dxp = rdx.data();
+    oracle_kronmult_adjoint_f64(6, 4, mats.data(), stride, &ip, &gp, &dxp, gm.data(), 1, 1);
+    double e1 = 0, m1 = 0, e2 = 0, m2 = 0;
+    for (int i = 0; i < S; ++i)
+    {
+        e1 = std::fmax(e1, std::fabs(dx[i] - rdx[i]));
+        m1 = std::fmax(m1, std::fabs(rdx[i]));
+    }
+    for (int m = 0; m < 6; ++m)
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c)
+            {
+                double const ref = rdA[m * 4 * stride + r + c * stride];
+                e2 = std::fmax(e2, std::fabs(dA[m][4 * r + c] - ref));
+                m2 = std::fmax(m2, std::fabs(ref));
+            }
+    std::printf("    d x max error %.2e of %.2e, dA max error %.2e of %.2e\n", e1, m1, e2, m2);
+    CHECK(e1 <= 1e-13 * m1 && e2 <= 1e-13 * m2, "adjoint2 emulation differs from the oracle adjoint\n");
+}
+
 int main()
 {
+    check_adjoint2();
     check_m8<3>("m8 d3");
     check_m8<4>("m8 d4");
     check<6, 4, 2>("d6n4r2", true, true);

@@ -57,7 +57,7 @@ def _check(b, gi, gm, want_input=True, want_mats=True):
 SHAPES = [(1, 1), (1, 5), (2, 3), (2, 8), (3, 5), (3, 8), (4, 4), (4, 8), (5, 3), (5, 4), (6, 2), (6, 4), (3, 7)]
 
 
-@pytest.mark.parametrize("intermediates", ["recompute", "store"])
+@pytest.mark.parametrize("intermediates", ["auto", "recompute", "store"])
 @pytest.mark.parametrize("d,n", SHAPES)
 def test_adjoint_random_f64(gpu, d, n, intermediates):
     nb = 37 if n ** d <= 4096 else 9
