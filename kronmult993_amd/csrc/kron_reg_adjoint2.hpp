@@ -9,8 +9,9 @@
 //     mode 1 contracted: X_0;
 //   * H chain, phase 2: g read in the phase-2 layout (elements j * 256 + t, coalesced); dA_0 from (H_0,
 //     X_0), A_0^T, dA_1 from (H_1, X_1), A_1^T, exchange into phase 1;
-//   * X_3 and X_2 rebuilt from x (modes 5, 4, exchange, mode 3), dA_2, A_2^T, dA_3, A_3^T, exchange into
-//     phase 0; X_5 = x and X_4 = A_5 x, dA_4, A_4^T, dA_5, A_5^T: d x in the phase-0 layout.
+//   * X_3 kept from the first pass (below), X_2 = A_3 X_3, dA_2, A_2^T, dA_3, A_3^T, exchange into phase 0;
+//     X_5 = x (read again) and X_4 (kept in FP32, A_5 x in FP64), dA_4, A_4^T, dA_5, A_5^T: d x in the
+//     phase-0 layout.
 // A factor gradient's per-thread partial (16 values: its 4 fibers of the mode) is reduce-scattered across
 // the wavefront in four lane-shuffle steps to one value per lane and summed over consecutive entries that
 // share the gradient pointer; when the pointer changes, two more steps, a sum over the four wavefronts in
@@ -69,38 +70,39 @@ __device__ __forceinline__ void contract(T (&v)[kR], T const *A_generic, int str
     }
 }
 
-// this thread's partial of dA_M: p[4 r + c] = sum over its 4 fibers f of h[f][r] x[f][c]
+// This thread's partial of dA_M, p[4 r + c] = sum over its 4 fibers f of h[f][r] x[f][c], reduce-scattered
+// across the wavefront: after step s (lane bit 5 - s) a lane keeps the half of its values selected by that
+// bit, summed with the partner lane's copy of the same half. Lane l ends with value index
+// 8 b5 + 4 b4 + 2 b3 + b2 (bits of l), summed over the 16 lanes that share its bits 0..1. The first step is
+// fused with the partial (values k and k + 8 formed together and folded at once, so the 16 are never live
+// together).
 template<typename T, int M>
-__device__ __forceinline__ void partial(T const (&h)[kR], T const (&x)[kR], T (&p)[16])
+__device__ __forceinline__ void partial_half(T const (&h)[kR], T const (&x)[kR], T (&q8)[8], int lane)
 {
     constexpr int S = stride_of(M);
-    constexpr int O = S == 4 ? 1 : 4;  // stride of the other register mode (the fiber index)
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-        {
-            T acc = T(0);
-#pragma unroll
-            for (int f = 0; f < 4; ++f) acc = fma(h[r * S + f * O], x[c * S + f * O], acc);
-            p[4 * r + c] = acc;
-        }
-}
-
-// 16 values per lane -> one per lane: after step s (lane bit 5 - s) a lane keeps the half of its values
-// selected by that bit, summed with the partner lane's copy of the same half. Lane l ends with value
-// index 8 b5 + 4 b4 + 2 b3 + b2 (bits of l), summed over the 16 lanes that share its bits 0..1.
-template<typename T>
-__device__ __forceinline__ T reduce_scatter(T (&p)[16], int lane)
-{
-    T q8[8], q4[4], q2[2];
-    bool const b5 = (lane >> 5) & 1, b4 = (lane >> 4) & 1, b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1;
+    constexpr int O = S == 4 ? 1 : 4;
+    bool const b5 = (lane >> 5) & 1;
 #pragma unroll
     for (int k = 0; k < 8; ++k)
     {
-        T const mine = b5 ? p[k + 8] : p[k], other = b5 ? p[k] : p[k + 8];
+        T lo = T(0), hi = T(0);
+        int const r0 = k >> 2, c0 = k & 3, r1 = (k + 8) >> 2, c1 = (k + 8) & 3;
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+        {
+            lo = fma(h[r0 * S + f * O], x[c0 * S + f * O], lo);
+            hi = fma(h[r1 * S + f * O], x[c1 * S + f * O], hi);
+        }
+        T const mine = b5 ? hi : lo, other = b5 ? lo : hi;
         q8[k] = mine + __shfl_xor(other, 32);
     }
+}
+
+template<typename T>
+__device__ __forceinline__ T reduce_scatter8(T (&q8)[8], int lane)
+{
+    T q4[4], q2[2];
+    bool const b4 = (lane >> 4) & 1, b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
     {
@@ -117,7 +119,7 @@ __device__ __forceinline__ T reduce_scatter(T (&p)[16], int lane)
     return mine + __shfl_xor(other, 4);
 }
 
-// add a run's dA partial (one value per lane, reduce_scatter order) to the gradient: lanes 0..3 of each
+// add a run's dA partial (one value per lane, in the order above) to the gradient: lanes 0..3 of each
 // quad hold partials of one index; the four wavefronts meet in red[]
 template<typename T>
 __device__ __forceinline__ void flush_gradient(T value, T *g_generic, int stride, T *red, int t)
@@ -163,11 +165,28 @@ __device__ __forceinline__ void load_x(T const *src_generic, T (&v)[kR], int t)
     for (int j = 0; j < kR; ++j) v[j] = src[t * kR + j];
 }
 
+// X_3 (phase-1 layout) and, in FP32, X_4 (phase-0 layout) are kept from the first pass of the X chain in
+// thread-private LDS slots (element j * P + t of the buffer: each thread reads back only what it wrote, so
+// no barrier) instead of being rebuilt from x; FP64 keeps X_3 only, as 64 KiB of LDS per workgroup already
+// matches its two resident workgroups per CU (223 VGPRs).
 template<typename T>
-struct Gradient
+constexpr bool keep_x4()
 {
-    T acc[6];
-};
+    return sizeof(T) == 4;
+}
+
+template<typename T>
+__device__ __forceinline__ void keep(T *buf, T const (&v)[kR], int t)
+{
+#pragma unroll
+    for (int j = 0; j < kR; ++j) buf[j * kP + t] = v[j];
+}
+template<typename T>
+__device__ __forceinline__ void restore(T const *buf, T (&v)[kR], int t)
+{
+#pragma unroll
+    for (int j = 0; j < kR; ++j) v[j] = buf[j * kP + t];
+}
 
 template<typename T>
 __global__ void __launch_bounds__(kP)
@@ -177,8 +196,12 @@ __global__ void __launch_bounds__(kP)
 {
     static_assert(PL::NPHASE == 3 && PL::elem(2, 1, 0) == 1 && PL::elem(2, 0, 1) == kP,
                   "phase 2 is element j * P + t");
-    __shared__ __attribute__((aligned(16))) T lds[PL::SIZE];
+    constexpr bool kKeep = true;
+    constexpr bool kKeep4 = keep_x4<T>();
+    __shared__ __attribute__((aligned(16))) T lds[PL::SIZE * (1 + (kKeep ? 1 : 0) + (kKeep4 ? 1 : 0))];
     __shared__ T red[kWaves * 16];
+    T *const x3buf = lds + PL::SIZE;
+    T *const x4buf = lds + 2 * PL::SIZE;
     int const t = threadIdx.x;
     int const lane = t & 63;
     int const e0 = kronreg::xcd_run_index() * epw;
@@ -197,12 +220,13 @@ __global__ void __launch_bounds__(kP)
     {
         bool const last = e + 1 >= e1;
         T const *const *A = mats + static_cast<size_t>(e) * 6;
-        T h[kR], v[kR], xs[kR], p[16];
+        T h[kR], v[kR], xs[kR];
         // one dA step: partial of mode M from (h, x), run sum, flush when the next entry's pointer differs
         auto grad = [&](auto mc, T const (&x)[kR]) {
             constexpr int M = decltype(mc)::value;
-            partial<T, M>(h, x, p);
-            gacc[M] += reduce_scatter(p, lane);
+            T q8[8];
+            partial_half<T, M>(h, x, q8, lane);
+            gacc[M] += reduce_scatter8(q8, lane);
             T *const gA = gmats[static_cast<size_t>(e) * 6 + M];
             if (last || gmats[static_cast<size_t>(e + 1) * 6 + M] != gA)
             {
@@ -220,8 +244,10 @@ __global__ void __launch_bounds__(kP)
         {
             load_x(in[e], v, t);
             contract<T, 5, false>(v, A[5], stride);
+            if constexpr (kKeep4) keep(x4buf, v, t);  // X_4
             contract<T, 4, false>(v, A[4], stride);
             exchange<T, 0, 1>(lds, v, t);
+            if constexpr (kKeep) keep(x3buf, v, t);   // X_3
             contract<T, 3, false>(v, A[3], stride);
             contract<T, 2, false>(v, A[2], stride);
             exchange<T, 1, 2>(lds, v, t);
@@ -236,10 +262,15 @@ __global__ void __launch_bounds__(kP)
         exchange<T, 2, 1>(lds, h, t);
         if (want_mats)
         {
-            load_x(in[e], v, t);
-            contract<T, 5, false>(v, A[5], stride);
-            contract<T, 4, false>(v, A[4], stride);
-            exchange<T, 0, 1>(lds, v, t);
+            if constexpr (kKeep)
+                restore(x3buf, v, t);
+            else
+            {
+                load_x(in[e], v, t);
+                contract<T, 5, false>(v, A[5], stride);
+                contract<T, 4, false>(v, A[4], stride);
+                exchange<T, 0, 1>(lds, v, t);
+            }
 #pragma unroll
             for (int j = 0; j < kR; ++j) xs[j] = v[j];  // X_3
             contract<T, 3, false>(v, A[3], stride);      // X_2
@@ -251,10 +282,15 @@ __global__ void __launch_bounds__(kP)
         exchange<T, 1, 0>(lds, h, t);
         if (want_mats)
         {
-            load_x(in[e], v, t);
+            load_x(in[e], xs, t);  // X_5
+            if constexpr (kKeep4)
+                restore(x4buf, v, t);
+            else
+            {
 #pragma unroll
-            for (int j = 0; j < kR; ++j) xs[j] = v[j];  // X_5
-            contract<T, 5, false>(v, A[5], stride);      // X_4
+                for (int j = 0; j < kR; ++j) v[j] = xs[j];
+                contract<T, 5, false>(v, A[5], stride);  // X_4
+            }
             grad(std::integral_constant<int, 4>(), v);
         }
         contract<T, 4, true>(h, A[4], stride);  // H_5

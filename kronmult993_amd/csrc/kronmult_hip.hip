@@ -1017,10 +1017,9 @@ hipError_t reg_adjoint_launch(Shape const &s, RegAdjMode mode, T const *const *m
     }
 }
 
-// d = 6, n = 4: the two-modes-per-phase adjoint (kron_reg_adjoint2.hpp), one run per resident workgroup
-#ifndef KRON_ADJ2
-#define KRON_ADJ2 1
-#endif
+// d = 6, n = 4: the two-modes-per-phase adjoint (kron_reg_adjoint2.hpp), one run per resident workgroup.
+// It serves AUTO (measured on the d=6 batch: FP64 258-266 us, FP32 142-145 us, against 522 / 304 us for
+// the one-mode-per-phase kernel, which RECOMPUTE and STORE keep).
 template<typename T>
 hipError_t adjoint2_launch(Shape const &s, T const *const *mats, T const *const *in, T const *const *gout,
                            T *const *gin, T *const *gmats, hipStream_t stream)
@@ -1041,7 +1040,7 @@ template<typename T>
 hipError_t reg_adjoint_dispatch(Shape const &s, int intermediates, T const *const *mats, T const *const *in,
                                 T const *const *gout, T *const *gin, T *const *gmats, hipStream_t stream)
 {
-    if (KRON_ADJ2 && s.d == 6 && s.n == 4 && intermediates == KRONMULT_ADJOINT_AUTO)
+    if (s.d == 6 && s.n == 4 && intermediates == KRONMULT_ADJOINT_AUTO)
         return adjoint2_launch<T>(s, mats, in, gout, gin, gmats, stream);
     // STORE: the chain in LDS where it fits comfortably (d=3 n=8, d=4 n=4), else in registers.
     // AUTO, measured per shape: d=4 n=8 and d=5/6 n=4 keep the chain in registers (d=6 n=4 FP64 0.58 ms
