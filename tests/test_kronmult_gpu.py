@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import accumulation_bound, normwise, shift_odd_inputs
+from helpers import grouped_by_output, accumulation_bound, normwise, shift_odd_inputs
 from kronmult993_amd import api, workloads as W
 from kronmult993_amd.device import DeviceBatch
 
@@ -230,6 +230,21 @@ def test_large_vectors_multipass(gpu, d, n, nb, dtype):
     _check_aliased(b, _run(b, gpu))
     u = _unaliased(b)
     _check_bitexact(u, _run(u, gpu))
+
+
+@pytest.mark.parametrize("d", [3, 4])
+@pytest.mark.parametrize("epw", [1, 3, 7])
+def test_matrix_core_n8_run_lengths(gpu, d, epw):
+    """The n = 8 FP64 matrix-core sweep with runs that start at odd entries (its alternating factor buffers)
+    and cut output segments at every place: grouped outputs of 1..5 entries, padded strides, element-aligned
+    odd inputs; bit-exact per entry."""
+    tu = {"family": api.FAMILY_REGISTER, "entries_per_workgroup": epw}
+    b = grouped_by_output(W.random_case(140 + d + epw, d, 8, 61, stride=11, distinct=17, shared_factors=5))
+    assert api.kernel_name(d, 8, 8, b.nb) == "reg_sweep_mfma_f64"
+    b = shift_odd_inputs(b)
+    _check_aliased(b, _run_tuned(b, gpu, tu))
+    u = _unaliased(b)
+    _check_bitexact(u, _run_tuned(u, gpu, tu))
 
 
 def test_empty_batch_and_invalid_arguments(gpu):
