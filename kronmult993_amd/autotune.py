@@ -20,7 +20,6 @@ from __future__ import annotations
 import json
 import math
 import os
-import statistics
 import threading
 from typing import Dict, List, Optional
 
@@ -102,18 +101,28 @@ class Autotuner:
 
     # ---- timing ----------------------------------------------------------------------------------------
     def _time_us(self, fn) -> float:
+        """Device time of one call of fn(stream): `reps` calls captured in one graph on a side stream, the
+        best of three replays, so the host's launch cost (a Python call per launch) is not timed."""
         torch = _torch()
-        fn()
-        torch.cuda.synchronize()
-        t = []
-        for _ in range(self.reps):
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
-            fn()
-            b.record()
-            b.synchronize()
-            t.append(a.elapsed_time(b) * 1e3)
-        return statistics.median(t)
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            fn(s)
+        s.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            for _ in range(self.reps):
+                fn(s)
+        best = float("inf")
+        with torch.cuda.stream(s):
+            for _ in range(3):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(s)
+                g.replay()
+                b.record(s)
+                b.synchronize()
+                best = min(best, a.elapsed_time(b) * 1e3 / self.reps)
+        return best
 
     # ---- one batch -------------------------------------------------------------------------------------
     def tune_single(self, db, force: bool = False) -> dict:
@@ -140,7 +149,7 @@ class Autotuner:
             pass
         best, best_t = None, float("inf")
         for c in cands:
-            t = self._time_us(lambda: db.run(sync=False, tuning=c))
+            t = self._time_us(lambda st: db.run(sync=False, tuning=c, stream=st))
             if t < best_t:
                 best, best_t = c, t
         choice = dict(best, us=round(best_t, 2))
@@ -164,20 +173,46 @@ class Autotuner:
         structs = api.grouped_structs(groups, dt)
         best = {"strategy": "separate", "workgroups_per_cu": 0, "entries_per_tile": [0] * len(dbs),
                 "tunings": tun_clean}
-        best_t = self._time_us(lambda: api.kronmult_batched_grouped(groups, dtype=dt, strategy="separate",
-                                                                    structs=structs))
-        # persistent launch: grid and tile length
+        best_t = self._time_us(lambda st: api.kronmult_batched_grouped(groups, dtype=dt, strategy="separate",
+                                                                       stream=st, structs=structs))
+        # persistent launch: grid and tile length, the same multiple of each group's pass for all groups ...
+        mults = (0.25, 0.5, 1, 2, 4)
+        tile = lambda db, m: max(1, int(entries_per_pass(db.d, db.n, db.itemsize) * m))
+
+        def time_persistent(wpc, epw):
+            groups = [db.group(tuning=t, entries_per_tile=e) for db, t, e in zip(dbs, tun_clean, epw)]
+            structs = api.grouped_structs(groups, dt)
+            return self._time_us(lambda st: api.kronmult_batched_grouped(
+                groups, dtype=dt, strategy="persistent", workgroups_per_cu=wpc, stream=st, structs=structs))
+
+        best_p, best_pt = None, float("inf")
         for wpc in (4, 8, 16, 32):
-            for mult in (0.25, 0.5, 1, 2, 4):
-                epw = [max(1, int(entries_per_pass(db.d, db.n, db.itemsize) * mult)) for db in dbs]
-                groups = [db.group(tuning=t, entries_per_tile=e) for db, t, e in zip(dbs, tun_clean, epw)]
-                structs = api.grouped_structs(groups, dt)
-                t = self._time_us(lambda: api.kronmult_batched_grouped(
-                    groups, dtype=dt, strategy="persistent", workgroups_per_cu=wpc, structs=structs))
-                if t < best_t:
-                    best_t = t
-                    best = {"strategy": "persistent", "workgroups_per_cu": wpc, "entries_per_tile": epw,
-                            "tunings": tun_clean}
+            for mult in mults:
+                epw = [tile(db, mult) for db in dbs]
+                t = time_persistent(wpc, epw)
+                if t < best_pt:
+                    best_pt, best_p = t, (wpc, list(epw))
+        # ... then each group's tile on its own, the others held (coordinate descent, at most two passes
+        # over the groups): the groups share the grid but not their tiles, and a short tile suits a group
+        # whose pass holds many entries
+        wpc, epw = best_p
+        for _ in range(2):
+            moved = False
+            for g, db in enumerate(dbs):
+                for mult in (0.125,) + mults:
+                    trial = list(epw)
+                    trial[g] = tile(db, mult)
+                    if trial[g] == epw[g]:
+                        continue
+                    t = time_persistent(wpc, trial)
+                    if t < best_pt:
+                        best_pt, epw, moved = t, trial, True
+            if not moved:
+                break
+        if best_pt < best_t:
+            best_t = best_pt
+            best = {"strategy": "persistent", "workgroups_per_cu": wpc, "entries_per_tile": epw,
+                    "tunings": tun_clean}
         choice = dict(best, us=round(best_t, 2))
         self._store(key, choice)
         return choice
