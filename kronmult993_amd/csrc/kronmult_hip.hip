@@ -695,6 +695,12 @@ size_t fill_group(Shape const &s, T const *const *mats, T const *const *in, T *c
 constexpr int kPackedMaxFibers = 64;        // at least 4 entries per 256-thread pass
 constexpr long kPackedLds = 40 * 1024;      // pass size for 4 resident workgroups per CU
 constexpr int kPackedWorkgroupsPerCu = 8;
+// Default tile: one pass, or, where that is at most half a pass, the tile that gives every CU
+// kPackedTilesPerCu tiles. The kernel waits on its staging loads, so the workgroups that share a CU hide
+// each other's latency; measured on MI355X (FP64): 20000 entries of d=2 n=8 26.8 -> 15.1 us, of d=2 n=6
+// 11.3 -> 9.5 us, of d=2 n=4 8.8 -> 7.2 us, 8192 of d=3 n=4 8.1 -> 7.4 us, 6000 of d=3 n=6 12.6 -> 11.1 us;
+// a tile above half a pass was slower than the full pass (20000 entries of d=5 n=2: 9.9 against 8.8 us).
+constexpr long kPackedTilesPerCu = 8;
 // The packed launch is the default from 20 entries per CU on: measured on MI355X, one workgroup per entry
 // is faster for 4096 entries of d=3 n=4 (6.2 vs 7.9 us), the packed launch for 6000 of d=3 n=6 (12.4 vs
 // 15.1 us) and 20000 of d=2 n=8 (26.5 vs 51 us).
@@ -717,6 +723,9 @@ hipError_t packed_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L,
     {
         ept = krongroup::kThreads / static_cast<int>(s.size / s.n);
         while (ept > 1 && ept * per_entry > kPackedLds) --ept;
+        long const tiles = kPackedTilesPerCu * device_cu_count();
+        long const short_tile = (s.nb + tiles - 1) / tiles;
+        if (2 * short_tile <= ept) ept = static_cast<int>(short_tile < 1 ? 1 : short_tile);
     }
     size_t const lds = fill_group<T>(s, mats, in, out, ept, table.g[0]);
     table.count = 1;
