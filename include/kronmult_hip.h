@@ -57,9 +57,10 @@ int kronmult_batched_f32_async(int matrix_count, int matrix_size, float const *c
  *                           workgroup, any n with n * sizeof(T) <= 32 KiB; n > 64 on the wide-fiber
  *                           kernel), _MATRIX (every contraction on the FP64 matrix cores;
  *                           d = 6, n = 4, double only) or _PACKED (fiber sweep with several small vectors per workgroup: n <= 8,
- *                           n^(d-1) <= 64; the default for such shapes from 20 entries per CU on);
- *   workgroups_per_cu:      workgroups to launch per compute unit (register, matrix, packed families);
- *   entries_per_workgroup:  consecutive entries one workgroup walks (register and matrix families), or
+ *                           n^(d-1) <= 64; the default for such shapes from 20 entries per CU on) or
+ *                           _WAVE (one wavefront per entry, one element per lane: n^d <= 64, 2 <= n <= 8);
+ *   workgroups_per_cu:      workgroups to launch per compute unit (register, matrix, packed, wave families);
+ *   entries_per_workgroup:  consecutive entries one workgroup walks (register, matrix, wave families), or
  *                           entries per tile (packed family).
  * An impossible request (a family that does not cover (d, n)) returns hipErrorInvalidValue.
  */
@@ -68,6 +69,7 @@ int kronmult_batched_f32_async(int matrix_count, int matrix_size, float const *c
 #define KRONMULT_FAMILY_FIBER 2
 #define KRONMULT_FAMILY_MATRIX 3
 #define KRONMULT_FAMILY_PACKED 4
+#define KRONMULT_FAMILY_WAVE 5
 
 typedef struct kronmult_tuning
 {
@@ -78,7 +80,7 @@ typedef struct kronmult_tuning
 
 typedef struct kronmult_launch_info
 {
-    int family;                /* KRONMULT_FAMILY_REGISTER, _FIBER, _MATRIX or _PACKED */
+    int family;                /* KRONMULT_FAMILY_REGISTER, _FIBER, _MATRIX, _PACKED or _WAVE */
     int grid;                  /* workgroups launched */
     int block;                 /* threads per workgroup */
     int entries_per_workgroup; /* entries walked by one workgroup */

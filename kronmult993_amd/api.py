@@ -83,7 +83,7 @@ def kronmult_batched_async(matrix_count: int, matrix_size: int, matrix_list_batc
         raise KronmultError(rc, "kronmult_batched_async")
 
 
-FAMILY_AUTO, FAMILY_REGISTER, FAMILY_FIBER, FAMILY_MATRIX, FAMILY_PACKED = 0, 1, 2, 3, 4
+FAMILY_AUTO, FAMILY_REGISTER, FAMILY_FIBER, FAMILY_MATRIX, FAMILY_PACKED, FAMILY_WAVE = 0, 1, 2, 3, 4, 5
 
 
 def _tuning(tuning) -> Optional[Tuning]:

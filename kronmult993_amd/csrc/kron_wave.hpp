@@ -1,0 +1,147 @@
+// kron_wave.hpp -- one wavefront per entry for small tensors (n^d <= 64, 2 <= n <= 8; no counterpart in
+// the reference, whose kernel gives every entry a workgroup of its own, kronmult_gpu/kronmult.cu:95-130).
+//
+// A workgroup is one wavefront and walks a run of consecutive entries. Lane l holds element l of the
+// entry (canonical order, mode d-1 fastest); lanes at or above n^d idle. A mode step writes the lanes'
+// values to the wavefront's LDS row, and lane l reads the n values of its fiber and row r(l) of the
+// factor (staged beside them, column-major) and forms the fused chain over k ascending from zero. The
+// wavefront's LDS instructions complete in program order, so there are no workgroup barriers: many
+// independent wavefronts per CU hide each other's memory latency. The next entry's element and factor
+// values are loaded one entry ahead. Consecutive entries with the same output pointer are summed per lane
+// and flushed with one atomic add per element and run. Modes d-1 .. 0, fused multiply-adds from zero:
+// bit-identical per entry to the reference's sweep (kronmult_gpu/kronmult.cu:66-70, :104).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kron_reg.hpp"
+
+namespace kronwave
+{
+
+constexpr int kLanes = 64;
+constexpr int kMaxD = 6;          // n >= 2 and n^d <= 64
+constexpr int kCoefPerLane = 2;   // d n^2 <= 128 for every such shape
+constexpr int kMaxCoef = kCoefPerLane * kLanes;
+
+// orders this wavefront's LDS accesses around it (a wavefront's LDS instructions complete in order; the
+// fences keep the compiler from moving accesses across)
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the shapes this kernel takes
+__host__ __device__ constexpr bool wave_shape(int d, int n)
+{
+    if (n < 2 || n > 8 || d < 1 || d > kMaxD) return false;
+    int size = 1;
+    for (int t = 0; t < d; ++t) size *= n;
+    return size <= kLanes && d * n * n <= kMaxCoef;
+}
+
+template<int D, int N>
+constexpr int wave_size()
+{
+    int s = 1;
+    for (int t = 0; t < D; ++t) s *= N;
+    return s;
+}
+
+// Entries [e0, e1) by this wavefront (lane = its lane index, xs / cs = its LDS rows of kLanes and kMaxCoef
+// elements). Used by the single-batch kernel below and by the grouped kernel (kron_group.hpp).
+template<typename T, int D, int N>
+__device__ __forceinline__ void wave_run(int e0, int e1, T const *const *__restrict__ mats, int stride,
+                                         T const *const *__restrict__ in, T *const *__restrict__ out, T *xs,
+                                         T *cs, int lane)
+{
+    static_assert(wave_shape(D, N), "n^d <= 64, 2 <= n <= 8");
+    constexpr int NN = N * N;
+    constexpr int NCOEF = D * NN;
+    constexpr int SIZE = wave_size<D, N>();
+    bool const live = lane < SIZE;
+    int const l = live ? lane : 0;
+
+    T x = T(0);
+    T f[kCoefPerLane];
+    auto load = [&](int e) {
+        if (live) x = kronreg::as_global(in[e])[lane];
+#pragma unroll
+        for (int c = 0; c < kCoefPerLane; ++c)
+        {
+            int const i = lane + c * kLanes;  // factor value i: A_m(r, k) at m n^2 + r + k n
+            if (i < NCOEF)
+            {
+                int const m = i / NN;
+                int const rk = i - m * NN;
+                int const k = rk / N;
+                int const r = rk - k * N;
+                f[c] = kronreg::as_global(mats[static_cast<size_t>(e) * D + m])[r + k * stride];
+            }
+        }
+    };
+    load(e0);
+
+    T run = T(0);
+    T *cur = out[e0];
+    for (int e = e0; e < e1; ++e)
+    {
+        xs[lane] = x;
+#pragma unroll
+        for (int c = 0; c < kCoefPerLane; ++c)
+        {
+            int const i = lane + c * kLanes;
+            if (i < NCOEF) cs[i] = f[c];
+        }
+        if (e + 1 < e1) load(e + 1);
+        wave_sync();
+        T y = T(0);
+        int st = 1;  // stride of mode t: N^(d-1-t)
+#pragma unroll
+        for (int t = D - 1; t >= 0; --t)
+        {
+            int const digit = (l / st) % N;
+            T const *xf = xs + (l - digit * st);
+            T const *a = cs + t * NN + digit;
+            T acc = T(0);
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc = fma(xf[k * st], a[k * N], acc);
+            y = acc;
+            wave_sync();
+            if (t > 0) xs[lane] = acc;  // after every lane's reads of this mode
+            wave_sync();
+            st *= N;
+        }
+        T *dst = out[e];
+        if (dst != cur)
+        {
+            if (live) atomicAdd((T *)(kronreg::as_global(cur) + lane), run);
+            run = T(0);
+            cur = dst;
+        }
+        run += y;
+    }
+    if (live) atomicAdd((T *)(kronreg::as_global(cur) + lane), run);
+}
+
+template<typename T, int D, int N>
+__global__ void __launch_bounds__(kLanes)
+    kron_wave_kernel(int nb, int epw, T const *const *__restrict__ mats, int stride, T const *const *__restrict__ in,
+                     T *const *__restrict__ out)
+{
+    __shared__ T xs[kLanes];
+    __shared__ T cs[kMaxCoef];
+    int const e0 = kronreg::xcd_run_index() * epw;
+    int const e1 = (e0 + epw < nb) ? e0 + epw : nb;
+    if (e0 >= e1) return;
+    wave_run<T, D, N>(e0, e1, mats, stride, in, out, xs, cs, static_cast<int>(threadIdx.x));
+}
+
+// The wave shapes, as X(D, N) for a macro X.
+#define KRON_WAVE_SHAPES(X)                                                                                      \
+    X(1, 2) X(2, 2) X(3, 2) X(4, 2) X(5, 2) X(6, 2) X(1, 3) X(2, 3) X(3, 3) X(1, 4) X(2, 4) X(3, 4) X(1, 5)       \
+        X(2, 5) X(1, 6) X(2, 6) X(1, 7) X(2, 7) X(1, 8) X(2, 8)
+
+} // namespace kronwave

@@ -37,6 +37,7 @@
 #include "kron_split_reg.hpp"
 #include "kron_reg.hpp"
 #include "kron_m8.hpp"
+#include "kron_wave.hpp"
 #include "kron_reg_adjoint.hpp"
 #include "kron_reg_adjoint2.hpp"
 #include "kronmult_hip.h"
@@ -747,6 +748,54 @@ hipError_t packed_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L,
     return hipGetLastError();
 }
 
+// ---- one wavefront per entry (n^d <= 64, 2 <= n <= 8; kron_wave.hpp) --------------------------------------
+// Default run length: enough runs for 8 workgroups per CU, at most 4 entries. Measured on MI355X (graph
+// replay, FP64): 20000 entries of d=2 n=8 7.9 us (packed launch 15.0), d=6 n=2 7.5 (14.0), d=2 n=4 5.8 (7.2);
+// 8192 of d=3 n=4 4.2 (packed 7.3), 4096 3.5 (fiber 6.3); 2000 of d=2 n=8 3.3 (fiber 7.1), 800 of d=6 n=2
+// 2.7 (fiber 3.5). Longer runs gave fewer workgroups than the latency needs, shorter ones more atomics.
+constexpr long kWaveRunsPerCu = 8;
+constexpr int kWaveMaxRun = 4;
+
+template<typename T, int D, int N>
+hipError_t wave_launch_dn(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T const *const *mats,
+                          T const *const *in, T *const *out, hipStream_t stream, bool run)
+{
+    auto kernel = kronwave::kron_wave_kernel<T, D, N>;
+    L.family = KRONMULT_FAMILY_WAVE;
+    L.block = kronwave::kLanes;
+    L.lds = sizeof(T) * (kronwave::kLanes + kronwave::kMaxCoef);
+    L.resident = resident_per_cu(kernel, kronwave::kLanes, 0);
+    long const cus = device_cu_count();
+    long epw = tu.entries_per_workgroup;
+    if (epw <= 0 && tu.workgroups_per_cu > 0)
+        epw = (s.nb + cus * tu.workgroups_per_cu - 1) / (cus * tu.workgroups_per_cu);
+    if (epw <= 0)
+    {
+        epw = (s.nb + cus * kWaveRunsPerCu - 1) / (cus * kWaveRunsPerCu);
+        if (epw > kWaveMaxRun) epw = kWaveMaxRun;
+    }
+    if (epw < 1) epw = 1;
+    L.epw = static_cast<int>(epw);
+    long const runs = (s.nb + epw - 1) / epw;
+    L.grid = (runs + 7) / 8 * 8;  // whole XCD stripes; surplus runs exit at once
+    if (L.grid > INT_MAX) return hipErrorInvalidValue;
+    if (!run || s.nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(L.grid)), dim3(kronwave::kLanes), 0, stream, s.nb, L.epw,
+                       mats, s.stride, in, out);
+    return hipGetLastError();
+}
+
+template<typename T>
+hipError_t wave_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T const *const *mats,
+                       T const *const *in, T *const *out, hipStream_t stream, bool run)
+{
+#define KRON_WAVE_LAUNCH(D, N) \
+    if (s.d == D && s.n == N) return wave_launch_dn<T, D, N>(s, tu, L, mats, in, out, stream, run);
+    KRON_WAVE_SHAPES(KRON_WAVE_LAUNCH)
+#undef KRON_WAVE_LAUNCH
+    return hipErrorInvalidValue;
+}
+
 // ---- entry --------------------------------------------------------------------------------------------
 template<typename T>
 int choose_family(Shape const &s, kronmult_tuning_t const &tu)
@@ -758,8 +807,10 @@ int choose_family(Shape const &s, kronmult_tuning_t const &tu)
     if (tu.family == KRONMULT_FAMILY_FIBER) return fiber_ok ? KRONMULT_FAMILY_FIBER : -1;
     if (tu.family == KRONMULT_FAMILY_MATRIX) return mfma_ok ? KRONMULT_FAMILY_MATRIX : -1;
     if (tu.family == KRONMULT_FAMILY_PACKED) return packed_ok<T>(s) ? KRONMULT_FAMILY_PACKED : -1;
+    if (tu.family == KRONMULT_FAMILY_WAVE) return kronwave::wave_shape(s.d, s.n) ? KRONMULT_FAMILY_WAVE : -1;
     if (tu.family != KRONMULT_FAMILY_AUTO) return -1;
     if (reg_ok) return KRONMULT_FAMILY_REGISTER;
+    if (kronwave::wave_shape(s.d, s.n)) return KRONMULT_FAMILY_WAVE;
     if (packed_ok<T>(s) && s.nb >= kPackedEntriesPerCu * device_cu_count()) return KRONMULT_FAMILY_PACKED;
     return fiber_ok ? KRONMULT_FAMILY_FIBER : -1;
 }
@@ -798,8 +849,11 @@ hipError_t kronmult_launch(int d, int n, T const *const *mats, int stride, T *co
     if (tu.workgroups_per_cu < 0 || tu.entries_per_workgroup < 0) return hipErrorInvalidValue;
     int family = choose_family<T>(s, tu);
     // the ordered epilogue exists for the register and one-entry-per-workgroup kernels
-    if (ordered && family == KRONMULT_FAMILY_PACKED && tu.family == KRONMULT_FAMILY_AUTO) family = KRONMULT_FAMILY_FIBER;
-    if (family < 0 || (ordered && (family == KRONMULT_FAMILY_MATRIX || family == KRONMULT_FAMILY_PACKED)))
+    if (ordered && (family == KRONMULT_FAMILY_PACKED || family == KRONMULT_FAMILY_WAVE) &&
+        tu.family == KRONMULT_FAMILY_AUTO)
+        family = KRONMULT_FAMILY_FIBER;
+    if (family < 0 || (ordered && (family == KRONMULT_FAMILY_MATRIX || family == KRONMULT_FAMILY_PACKED ||
+                                   family == KRONMULT_FAMILY_WAVE)))
         return hipErrorInvalidValue;
     bool const run = info == nullptr;
     if (run)
@@ -815,6 +869,8 @@ hipError_t kronmult_launch(int d, int n, T const *const *mats, int stride, T *co
         err = reg_dispatch<T>(s, tu, L, mats, cin, out, ws, stream, run);
     else if (family == KRONMULT_FAMILY_PACKED)
         err = packed_launch<T>(s, tu, L, mats, cin, out, stream, run);
+    else if (family == KRONMULT_FAMILY_WAVE)
+        err = wave_launch<T>(s, tu, L, mats, cin, out, stream, run);
     else if constexpr (sizeof(T) == 8)
         err = family == KRONMULT_FAMILY_MATRIX ? mfma_launch(s, tu, L, mats, cin, out, stream, run)
                                                : fiber_dispatch<T>(s, L, mats, cin, out, ws, stream, run);
@@ -889,6 +945,7 @@ char const *kernel_name(int d, int n, int nb)
     {
     case KRONMULT_FAMILY_MATRIX: return "mfma_sweep_f64";
     case KRONMULT_FAMILY_PACKED: return f64 ? "packed_f64" : "packed_f32";
+    case KRONMULT_FAMILY_WAVE: return f64 ? "wave_f64" : "wave_f32";
     case KRONMULT_FAMILY_REGISTER:
         if (m8_plan<T>(s.d, s.n)) return "reg_sweep_mfma_f64";
         return f64 ? "reg_sweep_f64" : "reg_sweep_f32";

@@ -317,13 +317,60 @@ def test_matrix_core_family_rejects_other_shapes(gpu):
 @pytest.mark.parametrize("d,n", [(3, 4), (2, 8), (3, 6), (5, 2), (6, 2), (4, 3), (1, 8), (2, 2), (3, 7)])
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_packed_family(gpu, d, n, dtype):
-    """Several small vectors per workgroup (the default for n^(d-1) <= 64 and large batches): padded strides,
-    shared factors, scattered aliasing; bit-identical per entry."""
+    """Several small vectors per workgroup (the default for n^(d-1) <= 64 and large batches of shapes above 64
+    elements; forced here for the wave shapes too): padded strides, shared factors, scattered aliasing;
+    bit-identical per entry."""
     b = W.random_case(120 + d * n, d, n, 6000, stride=n + 1, shared_factors=37).astype(dtype)
-    assert api.kernel_name(d, n, b.dtype.itemsize, b.nb).startswith("packed")
-    _check_aliased(b, _run(b, gpu))
+    wave = n ** d <= 64
+    assert api.kernel_name(d, n, b.dtype.itemsize, b.nb).startswith("wave" if wave else "packed")
+    tu = {"family": api.FAMILY_PACKED}
+    _check_aliased(b, _run_tuned(b, gpu, tu))
     u = _unaliased(b)
-    _check_bitexact(u, _run(u, gpu))
+    _check_bitexact(u, _run_tuned(u, gpu, tu))
+
+
+WAVE_SHAPES = [(1, 2), (2, 2), (3, 2), (4, 2), (5, 2), (6, 2), (1, 3), (2, 3), (3, 3), (1, 4), (2, 4), (3, 4),
+               (1, 5), (2, 5), (1, 6), (2, 6), (1, 7), (2, 7), (1, 8), (2, 8)]
+
+
+@pytest.mark.parametrize("d,n", WAVE_SHAPES)
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_wave_family(gpu, d, n, dtype):
+    """One wavefront per entry (n^d <= 64): padded strides, shared factors, scattered aliasing and runs of
+    equal outputs cut at every place (grouped outputs, 5 entries per wavefront); bit-identical per entry."""
+    tu = {"family": api.FAMILY_WAVE, "entries_per_workgroup": 5}
+    b = W.random_case(160 + d * n, d, n, 203, stride=n + 2, shared_factors=29).astype(dtype)
+    print(f"[case] wave d={d} n={n} {np.dtype(dtype).name} nb={b.nb}", flush=True)
+    _check_aliased(b, _run_tuned(b, gpu, tu))
+    g = shift_odd_inputs(grouped_by_output(W.random_case(170 + d * n, d, n, 97, distinct=23).astype(dtype)))
+    _check_aliased(g, _run_tuned(g, gpu, tu))
+    u = _unaliased(b)
+    _check_bitexact(u, _run_tuned(u, gpu, tu))
+
+
+@pytest.mark.parametrize("epw,wpc", [(1, 0), (3, 2), (0, 32), (64, 0)])
+def test_wave_family_launch_shapes(gpu, epw, wpc):
+    """Forced wave launches with explicit run lengths and grids; shapes outside the family are refused."""
+    tu = {"family": api.FAMILY_WAVE, "entries_per_workgroup": epw, "workgroups_per_cu": wpc}
+    b = W.random_case(181, 2, 8, 300, distinct=41)
+    _check_aliased(b, _run_tuned(b, gpu, tu))
+    u = _unaliased(b)
+    _check_bitexact(u, _run_tuned(u, gpu, tu))
+    for d, n in ((3, 5), (4, 4), (1, 9), (2, 1)):
+        with pytest.raises(api.KronmultError):
+            api.launch_info(d, n, 8, 100, {"family": api.FAMILY_WAVE})
+
+
+@pytest.mark.parametrize("d,n", [(3, 4), (2, 8), (6, 2), (1, 5)])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_fiber_family_small_shapes(gpu, d, n, dtype):
+    """The one-workgroup-per-entry fiber sweep, forced on shapes whose default is now the wave family."""
+    tu = {"family": api.FAMILY_FIBER}
+    b = W.random_case(190 + d * n, d, n, 150, stride=n + 1, shared_factors=11).astype(dtype)
+    assert api.launch_info(d, n, b.dtype.itemsize, b.nb, tu)["family"] == api.FAMILY_FIBER
+    _check_aliased(b, _run_tuned(b, gpu, tu))
+    u = _unaliased(b)
+    _check_bitexact(u, _run_tuned(u, gpu, tu))
 
 
 @pytest.mark.parametrize("ept,wpc", [(1, 0), (3, 2), (0, 64)])
