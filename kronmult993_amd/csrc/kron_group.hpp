@@ -16,16 +16,25 @@
 #include <hip/hip_runtime.h>
 
 #include "kron_reg.hpp"
+#include "kron_wave.hpp"
 
 namespace krongroup
 {
 
 constexpr int kMaxGroups = 16;
 constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kronwave::kLanes;
+
+// LDS elements of one wavefront's rows on the wave path
+__host__ __device__ constexpr int wave_lds_elems()
+{
+    return kronwave::kLanes + kronwave::kMaxCoef;
+}
 
 template<typename T>
 struct GroupDesc
 {
+    int wave;  // 1: the group's tiles run on the wave path (n^d <= 64, kron_wave.hpp)
     int d, n, stride, nb;
     int size;       // n^d
     int nfib;       // n^(d-1)
@@ -146,8 +155,12 @@ __device__ __forceinline__ void pass(GroupDesc<T> const &G, int e0, int cnt, T *
     __syncthreads();  // LDS free for the next pass
 }
 
-template<typename T>
-__global__ void __launch_bounds__(kThreads) kron_group_kernel(GroupTable<T> const table)
+// WAVE: the table may hold wave groups (the grouped launch). Their inlined paths take the kernel from 48 to
+// 82 VGPRs; the cap keeps 6 waves per SIMD without spills. The packed single-batch launch has no wave
+// groups and keeps 8 waves per SIMD.
+template<typename T, bool WAVE>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVE ? 6 : 1)))
+    kron_group_kernel(GroupTable<T> const table)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ T *out_slot[kThreads];  // a pass holds at most kThreads entries (one fiber each at least)
@@ -159,6 +172,30 @@ __global__ void __launch_bounds__(kThreads) kron_group_kernel(GroupTable<T> cons
         GroupDesc<T> const &G = table.g[gi];
         int const e_begin = (tile - table.tile_start[gi]) * G.epw;
         int const e_end = e_begin + G.epw < G.nb ? e_begin + G.epw : G.nb;
+        if (WAVE && G.wave)
+        {
+            // n^d <= 64: each wavefront sweeps a contiguous quarter of the tile on its own (kron_wave.hpp)
+            int const w = static_cast<int>(threadIdx.x) / kronwave::kLanes;
+            int const lane = static_cast<int>(threadIdx.x) - w * kronwave::kLanes;
+            int const chunk = (e_end - e_begin + kWaves - 1) / kWaves;
+            int const a = e_begin + w * chunk;
+            int const b = a + chunk < e_end ? a + chunk : e_end;
+            T *const xs = lds + w * wave_lds_elems();
+            T *const cs = xs + kronwave::kLanes;
+            if (a < b)
+            {
+                switch (G.d * 16 + G.n)
+                {
+#define KRON_GROUP_WAVE(D, N)                                                                          \
+    case D * 16 + N: kronwave::wave_run<T, D, N, true>(a, b, G.mats, G.stride, G.in, G.out, xs, cs, lane); break;
+                    KRON_WAVE_SHAPES(KRON_GROUP_WAVE)
+#undef KRON_GROUP_WAVE
+                default: break;
+                }
+            }
+            __syncthreads();  // LDS free for the next tile
+            continue;
+        }
         for (int e0 = e_begin; e0 < e_end; e0 += G.E)
         {
             int const cnt = e_end - e0 < G.E ? e_end - e0 : G.E;

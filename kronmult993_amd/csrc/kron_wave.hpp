@@ -52,7 +52,7 @@ constexpr int wave_size()
 
 // Entries [e0, e1) by this wavefront (lane = its lane index, xs / cs = its LDS rows of kLanes and kMaxCoef
 // elements). Used by the single-batch kernel below and by the grouped kernel (kron_group.hpp).
-template<typename T, int D, int N>
+template<typename T, int D, int N, bool OPAQUE_LANE = false>
 __device__ __forceinline__ void wave_run(int e0, int e1, T const *const *__restrict__ mats, int stride,
                                          T const *const *__restrict__ in, T *const *__restrict__ out, T *xs,
                                          T *cs, int lane)
@@ -61,6 +61,9 @@ __device__ __forceinline__ void wave_run(int e0, int e1, T const *const *__restr
     constexpr int NN = N * N;
     constexpr int NCOEF = D * NN;
     constexpr int SIZE = wave_size<D, N>();
+    // OPAQUE_LANE (the grouped kernel): inside its tile loop the compiler would otherwise hoist every
+    // shape's per-lane offsets out of the loop and keep them all live (215 VGPRs)
+    if constexpr (OPAQUE_LANE) asm volatile("" : "+v"(lane));
     bool const live = lane < SIZE;
     int const l = live ? lane : 0;
 

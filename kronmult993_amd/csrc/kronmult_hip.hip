@@ -669,8 +669,9 @@ bool group_persistent_ok(GroupT const &g, Shape const &s)
 // One group's descriptor; returns the LDS bytes its passes take.
 template<typename T>
 size_t fill_group(Shape const &s, T const *const *mats, T const *const *in, T *const *out, int entries_per_tile,
-                  krongroup::GroupDesc<T> &D)
+                  krongroup::GroupDesc<T> &D, bool wave = false)
 {
+    D.wave = wave && kronwave::wave_shape(s.d, s.n) ? 1 : 0;
     D.d = s.d;
     D.n = s.n;
     D.stride = s.stride;
@@ -688,6 +689,8 @@ size_t fill_group(Shape const &s, T const *const *mats, T const *const *in, T *c
     D.mats = mats;
     D.in = in;
     D.out = out;
+    // a wave group's tile is cut among the workgroup's wavefronts, each with its own LDS rows
+    if (D.wave) return krongroup::kWaves * krongroup::wave_lds_elems() * sizeof(T);
     return static_cast<size_t>(E) * per_entry + sizeof(void *);  // + pointer alignment
 }
 
@@ -741,9 +744,9 @@ hipError_t packed_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L,
     L.block = krongroup::kThreads;
     L.epw = table.g[0].epw;
     L.lds = lds;
-    L.resident = resident_per_cu(krongroup::kron_group_kernel<T>, krongroup::kThreads, lds);
+    L.resident = resident_per_cu(krongroup::kron_group_kernel<T, false>, krongroup::kThreads, lds);
     if (!run || grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(krongroup::kron_group_kernel<T>, dim3(static_cast<unsigned>(grid)), dim3(krongroup::kThreads),
+    hipLaunchKernelGGL((krongroup::kron_group_kernel<T, false>), dim3(static_cast<unsigned>(grid)), dim3(krongroup::kThreads),
                        lds, stream, table);
     return hipGetLastError();
 }
@@ -1256,6 +1259,7 @@ hipError_t kronmult_grouped_launch(GroupT const *groups, int count, kronmult_gro
     krongroup::GroupTable<T> table{};
     size_t lds = 0;
     int tiles = 0;
+    bool any_wave = false;
     for (int i = 0; i < count && i < krongroup::kMaxGroups; ++i)
     {
         if (!persistent[i]) continue;
@@ -1263,10 +1267,11 @@ hipError_t kronmult_grouped_launch(GroupT const *groups, int count, kronmult_gro
         Shape const &s = shapes[i];
         krongroup::GroupDesc<T> &D = table.g[table.count];
         size_t const need = fill_group<T>(s, g.matrix_list_batched, const_cast<T const *const *>(g.input_batched),
-                                          g.output_batched, g.entries_per_tile, D);
+                                          g.output_batched, g.entries_per_tile, D, true);
         table.tile_start[table.count] = tiles;
         tiles += (s.nb + D.epw - 1) / D.epw;
         if (need > lds) lds = need;
+        any_wave = any_wave || D.wave != 0;
         ++table.count;
     }
     table.tile_start[table.count] = tiles;
@@ -1274,8 +1279,8 @@ hipError_t kronmult_grouped_launch(GroupT const *groups, int count, kronmult_gro
     int const per_cu = opt.workgroups_per_cu > 0 ? opt.workgroups_per_cu : 2;
     long grid = static_cast<long>(device_cu_count()) * per_cu;
     if (grid > tiles) grid = tiles;
-    hipLaunchKernelGGL(krongroup::kron_group_kernel<T>, dim3(static_cast<unsigned>(grid)), dim3(krongroup::kThreads),
-                       lds, stream, table);
+    auto kernel = any_wave ? krongroup::kron_group_kernel<T, true> : krongroup::kron_group_kernel<T, false>;
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(grid)), dim3(krongroup::kThreads), lds, stream, table);
     return hipGetLastError();
 }
 
