@@ -28,7 +28,7 @@ constexpr int kWaves = kThreads / kronwave::kLanes;
 // LDS elements of one wavefront's rows on the wave path
 __host__ __device__ constexpr int wave_lds_elems()
 {
-    return kronwave::kLanes + kronwave::kMaxCoef;
+    return kronwave::kLanes + kronwave::kMaxCoef;  // one element per lane, d n^2 <= 128
 }
 
 template<typename T>

@@ -20,8 +20,9 @@ namespace kronwave
 {
 
 constexpr int kLanes = 64;
-constexpr int kMaxD = 6;          // n >= 2 and n^d <= 64
-constexpr int kCoefPerLane = 2;   // d n^2 <= 128 for every such shape
+constexpr int kMaxD = 8;          // n >= 2 and n^d <= kMaxElems
+constexpr int kMaxElems = 256;    // elements per entry: at most 4 per lane
+constexpr int kCoefPerLane = 2;   // d n^2 <= 128 for every one-element-per-lane shape
 constexpr int kMaxCoef = kCoefPerLane * kLanes;
 
 // orders this wavefront's LDS accesses around it (a wavefront's LDS instructions complete in order; the
@@ -33,46 +34,61 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// the shapes this kernel takes
-__host__ __device__ constexpr bool wave_shape(int d, int n)
+__host__ __device__ constexpr int wave_elems(int d, int n)
 {
-    if (n < 2 || n > 8 || d < 1 || d > kMaxD) return false;
     int size = 1;
     for (int t = 0; t < d; ++t) size *= n;
-    return size <= kLanes && d * n * n <= kMaxCoef;
+    return size;
+}
+
+// the shapes the one-element-per-lane path takes (the grouped kernel's wave groups)
+__host__ __device__ constexpr bool wave_shape(int d, int n)
+{
+    return n >= 2 && n <= 8 && d >= 1 && d <= kMaxD && wave_elems(d, n) <= kLanes && d * n * n <= kMaxCoef;
 }
 
 template<int D, int N>
 constexpr int wave_size()
 {
-    int s = 1;
-    for (int t = 0; t < D; ++t) s *= N;
-    return s;
+    return wave_elems(D, N);
 }
 
-// Entries [e0, e1) by this wavefront (lane = its lane index, xs / cs = its LDS rows of kLanes and kMaxCoef
-// elements). Used by the single-batch kernel below and by the grouped kernel (kron_group.hpp).
+// LDS elements of one wavefront's rows for (D, N): the tensor and the entry's factors, whole lanes each
+template<int D, int N>
+constexpr int wave_lds()
+{
+    return (wave_size<D, N>() + kLanes - 1) / kLanes * kLanes + (D * N * N + kLanes - 1) / kLanes * kLanes;
+}
+
+// Entries [e0, e1) by this wavefront (lane = its lane index, xs / cs = its LDS rows: R = ceil(n^d / 64)
+// lanes' worth of tensor and ceil(d n^2 / 64) of factors). Lane l holds elements l + 64 j, j < R. Used by
+// the single-batch kernel below and by the grouped kernel (kron_group.hpp, one element per lane).
 template<typename T, int D, int N, bool OPAQUE_LANE = false>
 __device__ __forceinline__ void wave_run(int e0, int e1, T const *const *__restrict__ mats, int stride,
                                          T const *const *__restrict__ in, T *const *__restrict__ out, T *xs,
                                          T *cs, int lane)
 {
-    static_assert(wave_shape(D, N), "n^d <= 64, 2 <= n <= 8");
     constexpr int NN = N * N;
     constexpr int NCOEF = D * NN;
     constexpr int SIZE = wave_size<D, N>();
+    constexpr int R = (SIZE + kLanes - 1) / kLanes;
+    constexpr int CPL = (NCOEF + kLanes - 1) / kLanes;
+    static_assert(N >= 2 && N <= 8 && SIZE <= kMaxElems, "2 <= n <= 8, n^d <= 256");
     // OPAQUE_LANE (the grouped kernel): inside its tile loop the compiler would otherwise hoist every
     // shape's per-lane offsets out of the loop and keep them all live (215 VGPRs)
     if constexpr (OPAQUE_LANE) asm volatile("" : "+v"(lane));
-    bool const live = lane < SIZE;
-    int const l = live ? lane : 0;
 
-    T x = T(0);
-    T f[kCoefPerLane];
+    T x[R];
+    T f[CPL];
     auto load = [&](int e) {
-        if (live) x = kronreg::as_global(in[e])[lane];
 #pragma unroll
-        for (int c = 0; c < kCoefPerLane; ++c)
+        for (int j = 0; j < R; ++j)
+        {
+            int const idx = lane + j * kLanes;
+            if (idx < SIZE) x[j] = kronreg::as_global(in[e])[idx];
+        }
+#pragma unroll
+        for (int c = 0; c < CPL; ++c)
         {
             int const i = lane + c * kLanes;  // factor value i: A_m(r, k) at m n^2 + r + k n
             if (i < NCOEF)
@@ -85,48 +101,74 @@ __device__ __forceinline__ void wave_run(int e0, int e1, T const *const *__restr
             }
         }
     };
+#pragma unroll
+    for (int j = 0; j < R; ++j) x[j] = T(0);
     load(e0);
 
-    T run = T(0);
+    T run[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) run[j] = T(0);
     T *cur = out[e0];
     for (int e = e0; e < e1; ++e)
     {
-        xs[lane] = x;
 #pragma unroll
-        for (int c = 0; c < kCoefPerLane; ++c)
+        for (int j = 0; j < R; ++j) xs[lane + j * kLanes] = x[j];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c)
         {
             int const i = lane + c * kLanes;
             if (i < NCOEF) cs[i] = f[c];
         }
         if (e + 1 < e1) load(e + 1);
         wave_sync();
-        T y = T(0);
+        T y[R];
         int st = 1;  // stride of mode t: N^(d-1-t)
 #pragma unroll
         for (int t = D - 1; t >= 0; --t)
         {
-            int const digit = (l / st) % N;
-            T const *xf = xs + (l - digit * st);
-            T const *a = cs + t * NN + digit;
-            T acc = T(0);
 #pragma unroll
-            for (int k = 0; k < N; ++k) acc = fma(xf[k * st], a[k * N], acc);
-            y = acc;
+            for (int j = 0; j < R; ++j)
+            {
+                int const idx = lane + j * kLanes;
+                int const l = idx < SIZE ? idx : 0;
+                int const digit = (l / st) % N;
+                T const *xf = xs + (l - digit * st);
+                T const *a = cs + t * NN + digit;
+                T acc = T(0);
+#pragma unroll
+                for (int k = 0; k < N; ++k) acc = fma(xf[k * st], a[k * N], acc);
+                y[j] = acc;
+            }
             wave_sync();
-            if (t > 0) xs[lane] = acc;  // after every lane's reads of this mode
+            if (t > 0)
+            {
+#pragma unroll
+                for (int j = 0; j < R; ++j) xs[lane + j * kLanes] = y[j];  // after every lane's reads
+            }
             wave_sync();
             st *= N;
         }
         T *dst = out[e];
         if (dst != cur)
         {
-            if (live) atomicAdd((T *)(kronreg::as_global(cur) + lane), run);
-            run = T(0);
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+            {
+                int const idx = lane + j * kLanes;
+                if (idx < SIZE) atomicAdd((T *)(kronreg::as_global(cur) + idx), run[j]);
+                run[j] = T(0);
+            }
             cur = dst;
         }
-        run += y;
+#pragma unroll
+        for (int j = 0; j < R; ++j) run[j] += y[j];
     }
-    if (live) atomicAdd((T *)(kronreg::as_global(cur) + lane), run);
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+    {
+        int const idx = lane + j * kLanes;
+        if (idx < SIZE) atomicAdd((T *)(kronreg::as_global(cur) + idx), run[j]);
+    }
 }
 
 template<typename T, int D, int N>
@@ -134,17 +176,27 @@ __global__ void __launch_bounds__(kLanes)
     kron_wave_kernel(int nb, int epw, T const *const *__restrict__ mats, int stride, T const *const *__restrict__ in,
                      T *const *__restrict__ out)
 {
-    __shared__ T xs[kLanes];
-    __shared__ T cs[kMaxCoef];
+    __shared__ T rows[wave_lds<D, N>()];
     int const e0 = kronreg::xcd_run_index() * epw;
     int const e1 = (e0 + epw < nb) ? e0 + epw : nb;
     if (e0 >= e1) return;
-    wave_run<T, D, N>(e0, e1, mats, stride, in, out, xs, cs, static_cast<int>(threadIdx.x));
+    constexpr int XS = (wave_size<D, N>() + kLanes - 1) / kLanes * kLanes;
+    wave_run<T, D, N>(e0, e1, mats, stride, in, out, rows, rows + XS, static_cast<int>(threadIdx.x));
 }
 
-// The wave shapes, as X(D, N) for a macro X.
+// The one-element-per-lane shapes (grouped kernel and single batch), as X(D, N) for a macro X.
 #define KRON_WAVE_SHAPES(X)                                                                                      \
     X(1, 2) X(2, 2) X(3, 2) X(4, 2) X(5, 2) X(6, 2) X(1, 3) X(2, 3) X(3, 3) X(1, 4) X(2, 4) X(3, 4) X(1, 5)       \
         X(2, 5) X(1, 6) X(2, 6) X(1, 7) X(2, 7) X(1, 8) X(2, 8)
+// Shapes with 2 to 4 elements per lane and no register-resident plan (single batch only).
+#define KRON_WAVE_SHAPES_WIDE(X) X(7, 2) X(8, 2) X(4, 3) X(5, 3) X(3, 5) X(3, 6)
+
+// whether (d, n) is one of the shapes above
+__host__ __device__ constexpr bool wave_family_shape(int d, int n)
+{
+    if (wave_shape(d, n)) return true;
+    return (d == 7 && n == 2) || (d == 8 && n == 2) || (d == 4 && n == 3) || (d == 5 && n == 3) ||
+           (d == 3 && n == 5) || (d == 3 && n == 6);
+}
 
 } // namespace kronwave

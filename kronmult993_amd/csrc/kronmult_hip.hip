@@ -758,6 +758,10 @@ hipError_t packed_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L,
 // 2.7 (fiber 3.5). Longer runs gave fewer workgroups than the latency needs, shorter ones more atomics.
 constexpr long kWaveRunsPerCu = 8;
 constexpr int kWaveMaxRun = 4;
+// 2 to 4 elements per lane: 20000 entries of d=7 n=2 13.6 us (packed launch 27.1), d=8 n=2 27.5 (fiber 44.0),
+// d=5 n=3 24.7 (fiber 47.3), d=4 n=3 10.7 (packed 13.5), d=3 n=5 14.2 (packed 17.4), d=3 n=6 29.3 (packed
+// 29.7) at runs of 2; runs of 4 were 1-7% slower, runs of 1 10-50% slower.
+constexpr int kWaveMaxRunWide = 2;
 
 template<typename T, int D, int N>
 hipError_t wave_launch_dn(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T const *const *mats,
@@ -766,7 +770,7 @@ hipError_t wave_launch_dn(Shape const &s, kronmult_tuning_t const &tu, Launch &L
     auto kernel = kronwave::kron_wave_kernel<T, D, N>;
     L.family = KRONMULT_FAMILY_WAVE;
     L.block = kronwave::kLanes;
-    L.lds = sizeof(T) * (kronwave::kLanes + kronwave::kMaxCoef);
+    L.lds = sizeof(T) * kronwave::wave_lds<D, N>();
     L.resident = resident_per_cu(kernel, kronwave::kLanes, 0);
     long const cus = device_cu_count();
     long epw = tu.entries_per_workgroup;
@@ -775,7 +779,8 @@ hipError_t wave_launch_dn(Shape const &s, kronmult_tuning_t const &tu, Launch &L
     if (epw <= 0)
     {
         epw = (s.nb + cus * kWaveRunsPerCu - 1) / (cus * kWaveRunsPerCu);
-        if (epw > kWaveMaxRun) epw = kWaveMaxRun;
+        int const max_run = kronwave::wave_size<D, N>() <= kronwave::kLanes ? kWaveMaxRun : kWaveMaxRunWide;
+        if (epw > max_run) epw = max_run;
     }
     if (epw < 1) epw = 1;
     L.epw = static_cast<int>(epw);
@@ -795,6 +800,7 @@ hipError_t wave_launch(Shape const &s, kronmult_tuning_t const &tu, Launch &L, T
 #define KRON_WAVE_LAUNCH(D, N) \
     if (s.d == D && s.n == N) return wave_launch_dn<T, D, N>(s, tu, L, mats, in, out, stream, run);
     KRON_WAVE_SHAPES(KRON_WAVE_LAUNCH)
+    KRON_WAVE_SHAPES_WIDE(KRON_WAVE_LAUNCH)
 #undef KRON_WAVE_LAUNCH
     return hipErrorInvalidValue;
 }
@@ -810,10 +816,12 @@ int choose_family(Shape const &s, kronmult_tuning_t const &tu)
     if (tu.family == KRONMULT_FAMILY_FIBER) return fiber_ok ? KRONMULT_FAMILY_FIBER : -1;
     if (tu.family == KRONMULT_FAMILY_MATRIX) return mfma_ok ? KRONMULT_FAMILY_MATRIX : -1;
     if (tu.family == KRONMULT_FAMILY_PACKED) return packed_ok<T>(s) ? KRONMULT_FAMILY_PACKED : -1;
-    if (tu.family == KRONMULT_FAMILY_WAVE) return kronwave::wave_shape(s.d, s.n) ? KRONMULT_FAMILY_WAVE : -1;
+    if (tu.family == KRONMULT_FAMILY_WAVE) return kronwave::wave_family_shape(s.d, s.n) ? KRONMULT_FAMILY_WAVE : -1;
     if (tu.family != KRONMULT_FAMILY_AUTO) return -1;
     if (reg_ok) return KRONMULT_FAMILY_REGISTER;
-    if (kronwave::wave_shape(s.d, s.n)) return KRONMULT_FAMILY_WAVE;
+    // d=3 n=6 FP64 (4 elements per lane, 152 VGPRs) measured 3-4% behind the packed launch
+    bool const wave_default = kronwave::wave_family_shape(s.d, s.n) && !(sizeof(T) == 8 && s.d == 3 && s.n == 6);
+    if (wave_default) return KRONMULT_FAMILY_WAVE;
     if (packed_ok<T>(s) && s.nb >= kPackedEntriesPerCu * device_cu_count()) return KRONMULT_FAMILY_PACKED;
     return fiber_ok ? KRONMULT_FAMILY_FIBER : -1;
 }

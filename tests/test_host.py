@@ -130,11 +130,13 @@ def test_kernel_selection():
     assert api.kernel_name(4, 8, 8, 16384) == "reg_sweep_mfma_f64"  # n = 8 FP64: matrix-core sweep
     assert api.kernel_name(3, 8, 8, 8192) == "reg_sweep_mfma_f64"
     assert api.kernel_name(3, 8, 4, 8192) == "reg_sweep_f32"
-    assert api.kernel_name(3, 6, 8, 6000) == "packed_f64"      # n^(d-1) = 36 <= 64 fibers, large batch
+    assert api.kernel_name(3, 7, 8, 6000) == "packed_f64"      # n^(d-1) = 49 <= 64 fibers, large batch
+    assert api.kernel_name(3, 6, 4, 6000) == "wave_f32"        # 216 elements: 4 per lane of one wavefront
+    assert api.kernel_name(3, 6, 8, 6000) == "packed_f64"      # ... except in FP64 (152 VGPRs), measured slower
     assert api.kernel_name(3, 4, 8, 4096) == "wave_f64"        # n^d <= 64: one wavefront per entry
     assert api.kernel_name(2, 8, 8, 20000) == "wave_f64"
     assert api.kernel_name(6, 2, 4, 800) == "wave_f32"
-    assert api.kernel_name(3, 6, 8, 100).startswith("fiber_lds")  # small batch: one workgroup per entry
+    assert api.kernel_name(3, 7, 8, 100).startswith("fiber_lds")  # small batch: one workgroup per entry
     assert api.kernel_name(4, 5, 4, 6000).startswith("fiber_lds")  # 125 fibers: fills a workgroup alone
     assert api.kernel_name(6, 8, 8, 3903) == "multipass_f64"
     assert api.kernel_name(3, 21, 8, 6) == "fiber_global_f64"

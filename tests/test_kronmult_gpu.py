@@ -321,7 +321,7 @@ def test_packed_family(gpu, d, n, dtype):
     elements; forced here for the wave shapes too): padded strides, shared factors, scattered aliasing;
     bit-identical per entry."""
     b = W.random_case(120 + d * n, d, n, 6000, stride=n + 1, shared_factors=37).astype(dtype)
-    wave = n ** d <= 64
+    wave = (d, n) in WAVE_SHAPES and not ((d, n) == (3, 6) and dtype == np.float64)
     assert api.kernel_name(d, n, b.dtype.itemsize, b.nb).startswith("wave" if wave else "packed")
     tu = {"family": api.FAMILY_PACKED}
     _check_aliased(b, _run_tuned(b, gpu, tu))
@@ -330,13 +330,14 @@ def test_packed_family(gpu, d, n, dtype):
 
 
 WAVE_SHAPES = [(1, 2), (2, 2), (3, 2), (4, 2), (5, 2), (6, 2), (1, 3), (2, 3), (3, 3), (1, 4), (2, 4), (3, 4),
-               (1, 5), (2, 5), (1, 6), (2, 6), (1, 7), (2, 7), (1, 8), (2, 8)]
+               (1, 5), (2, 5), (1, 6), (2, 6), (1, 7), (2, 7), (1, 8), (2, 8),
+               (7, 2), (8, 2), (4, 3), (5, 3), (3, 5), (3, 6)]  # 2 to 4 elements per lane
 
 
 @pytest.mark.parametrize("d,n", WAVE_SHAPES)
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_wave_family(gpu, d, n, dtype):
-    """One wavefront per entry (n^d <= 64): padded strides, shared factors, scattered aliasing and runs of
+    """One wavefront per entry (n^d <= 256, up to 4 elements per lane): padded strides, shared factors, scattered aliasing and runs of
     equal outputs cut at every place (grouped outputs, 5 entries per wavefront); bit-identical per entry."""
     tu = {"family": api.FAMILY_WAVE, "entries_per_workgroup": 5}
     b = W.random_case(160 + d * n, d, n, 203, stride=n + 2, shared_factors=29).astype(dtype)
@@ -356,12 +357,12 @@ def test_wave_family_launch_shapes(gpu, epw, wpc):
     _check_aliased(b, _run_tuned(b, gpu, tu))
     u = _unaliased(b)
     _check_bitexact(u, _run_tuned(u, gpu, tu))
-    for d, n in ((3, 5), (4, 4), (1, 9), (2, 1)):
+    for d, n in ((3, 7), (4, 4), (1, 9), (2, 1), (6, 3)):
         with pytest.raises(api.KronmultError):
             api.launch_info(d, n, 8, 100, {"family": api.FAMILY_WAVE})
 
 
-@pytest.mark.parametrize("d,n", [(3, 4), (2, 8), (6, 2), (1, 5)])
+@pytest.mark.parametrize("d,n", [(3, 4), (2, 8), (6, 2), (1, 5), (5, 3)])
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_fiber_family_small_shapes(gpu, d, n, dtype):
     """The one-workgroup-per-entry fiber sweep, forced on shapes whose default is now the wave family."""
