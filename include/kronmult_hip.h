@@ -58,7 +58,8 @@ int kronmult_batched_f32_async(int matrix_count, int matrix_size, float const *c
  *                           kernel), _MATRIX (every contraction on the FP64 matrix cores;
  *                           d = 6, n = 4, double only) or _PACKED (fiber sweep with several small vectors per workgroup: n <= 8,
  *                           n^(d-1) <= 64; the default for such shapes from 20 entries per CU on) or
- *                           _WAVE (one wavefront per entry, one element per lane: n^d <= 64, 2 <= n <= 8);
+ *                           _WAVE (one wavefront per entry: n^d <= 64 with 2 <= n <= 8, one element per lane,
+ *                           and d=7,8 n=2; d=4,5 n=3; d=3 n=5,6; d=2 n=9,10 with up to four);
  *   workgroups_per_cu:      workgroups to launch per compute unit (register, matrix, packed, wave families);
  *   entries_per_workgroup:  consecutive entries one workgroup walks (register, matrix, wave families), or
  *                           entries per tile (packed family).

@@ -73,7 +73,7 @@ __device__ __forceinline__ void wave_run(int e0, int e1, T const *const *__restr
     constexpr int SIZE = wave_size<D, N>();
     constexpr int R = (SIZE + kLanes - 1) / kLanes;
     constexpr int CPL = (NCOEF + kLanes - 1) / kLanes;
-    static_assert(N >= 2 && N <= 8 && SIZE <= kMaxElems, "2 <= n <= 8, n^d <= 256");
+    static_assert(N >= 2 && N <= 10 && SIZE <= kMaxElems, "2 <= n <= 10, n^d <= 256");
     // OPAQUE_LANE (the grouped kernel): inside its tile loop the compiler would otherwise hoist every
     // shape's per-lane offsets out of the loop and keep them all live (215 VGPRs)
     if constexpr (OPAQUE_LANE) asm volatile("" : "+v"(lane));
@@ -189,14 +189,14 @@ __global__ void __launch_bounds__(kLanes)
     X(1, 2) X(2, 2) X(3, 2) X(4, 2) X(5, 2) X(6, 2) X(1, 3) X(2, 3) X(3, 3) X(1, 4) X(2, 4) X(3, 4) X(1, 5)       \
         X(2, 5) X(1, 6) X(2, 6) X(1, 7) X(2, 7) X(1, 8) X(2, 8)
 // Shapes with 2 to 4 elements per lane and no register-resident plan (single batch only).
-#define KRON_WAVE_SHAPES_WIDE(X) X(7, 2) X(8, 2) X(4, 3) X(5, 3) X(3, 5) X(3, 6)
+#define KRON_WAVE_SHAPES_WIDE(X) X(7, 2) X(8, 2) X(4, 3) X(5, 3) X(3, 5) X(3, 6) X(2, 9) X(2, 10)
 
 // whether (d, n) is one of the shapes above
 __host__ __device__ constexpr bool wave_family_shape(int d, int n)
 {
     if (wave_shape(d, n)) return true;
     return (d == 7 && n == 2) || (d == 8 && n == 2) || (d == 4 && n == 3) || (d == 5 && n == 3) ||
-           (d == 3 && n == 5) || (d == 3 && n == 6);
+           (d == 3 && n == 5) || (d == 3 && n == 6) || (d == 2 && n == 9) || (d == 2 && n == 10);
 }
 
 } // namespace kronwave

@@ -331,7 +331,7 @@ def test_packed_family(gpu, d, n, dtype):
 
 WAVE_SHAPES = [(1, 2), (2, 2), (3, 2), (4, 2), (5, 2), (6, 2), (1, 3), (2, 3), (3, 3), (1, 4), (2, 4), (3, 4),
                (1, 5), (2, 5), (1, 6), (2, 6), (1, 7), (2, 7), (1, 8), (2, 8),
-               (7, 2), (8, 2), (4, 3), (5, 3), (3, 5), (3, 6)]  # 2 to 4 elements per lane
+               (7, 2), (8, 2), (4, 3), (5, 3), (3, 5), (3, 6), (2, 9), (2, 10)]  # 2 to 4 elements per lane
 
 
 @pytest.mark.parametrize("d,n", WAVE_SHAPES)
