@@ -143,6 +143,12 @@ class Autotuner:
         except api.KronmultError:
             pass
         try:
+            api.launch_info(db.d, db.n, db.itemsize, db.nb, {"family": api.FAMILY_WAVE})
+            cands += [{"family": api.FAMILY_WAVE}] + [{"family": api.FAMILY_WAVE, "entries_per_workgroup": e}
+                                                      for e in (1, 2, 4)]
+        except api.KronmultError:
+            pass
+        try:
             api.launch_info(db.d, db.n, db.itemsize, db.nb, {"family": api.FAMILY_MATRIX})
             cands += [{"family": api.FAMILY_MATRIX, "workgroups_per_cu": w} for w in (2, 3, 4)]
         except api.KronmultError:
