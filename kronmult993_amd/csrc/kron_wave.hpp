@@ -1,9 +1,10 @@
-// kron_wave.hpp -- one wavefront per entry for small tensors (n^d <= 64, 2 <= n <= 8; no counterpart in
-// the reference, whose kernel gives every entry a workgroup of its own, kronmult_gpu/kronmult.cu:95-130).
+// kron_wave.hpp -- one wavefront per entry for small tensors (n^d <= 64 with 2 <= n <= 8, and a few shapes
+// of up to 256 elements; no counterpart in the reference, whose kernel gives every entry a workgroup of its
+// own, kronmult_gpu/kronmult.cu:95-130).
 //
-// A workgroup is one wavefront and walks a run of consecutive entries. Lane l holds element l of the
-// entry (canonical order, mode d-1 fastest); lanes at or above n^d idle. A mode step writes the lanes'
-// values to the wavefront's LDS row, and lane l reads the n values of its fiber and row r(l) of the
+// A workgroup is one wavefront and walks a run of consecutive entries. Lane l holds elements l + 64 j of
+// the entry (canonical order, mode d-1 fastest); lanes past n^d idle. A mode step writes the lanes'
+// values to the wavefront's LDS row, and lane l reads the n values of each of its fibers and row r of the
 // factor (staged beside them, column-major) and forms the fused chain over k ascending from zero. The
 // wavefront's LDS instructions complete in program order, so there are no workgroup barriers: many
 // independent wavefronts per CU hide each other's memory latency. The next entry's element and factor
